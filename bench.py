@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""fwd+back GUPS of the gfx950 cone-beam projector pair on one MI355X.
+
+Workload (default): 512^3 volume at 0.5 mm, 720 views over 360 degrees,
+1024x768 flat detector at 0.39 mm, SAD 1000 mm, SDD 1500 mm, seeded
+ellipsoid phantom (synthetic).  One step = one forward projection of the
+volume onto all 720 views + one backprojection of all 720 views, inputs
+resident in HBM.  GUPS = 2 * Nvox * Nviews / t_step / 1e9.
+
+Prints one JSON line (rank 0) with the roofline of the dominant kernel and
+a CPU baseline (FP32 OpenMP restatement timed on this host on a bounded
+sample of views).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gt-tomography_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(kind, nvox, nviews, nrays, nsamp, s_vol, s_proj):
+    """Footprint-convention bytes of one launch.
+    FP: every sample reads its 8-tap footprint, every ray writes one value.
+    BP: every voxel-view reads its 4-tap footprint, every voxel is written."""
+    if kind == "fp":
+        return nrays * nsamp * 8 * s_vol + nrays * 4
+    return nvox * nviews * 4 * s_proj + nvox * 4
+
+
+def cpu_baseline(cfg, vol_host, angles, budget_s=12.0):
+    """FP32 OpenMP restatement (oracle/libcpucbct.so) on a bounded sample:
+    forward and back projection of the full volume for the first n views."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as O
+    g = cfg.geom()
+    og = O.Geom(g.nx, g.ny, g.nz, g.lx, g.ly, g.lz, g.nu, g.nv, g.fov_u, g.fov_v, g.sad, g.sdd)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def run(n):
+        a = angles[:n]
+        t0 = time.perf_counter()
+        p = O.cpu_fp_f32(og, a, None, vol_host)
+        t1 = time.perf_counter()
+        O.cpu_bp_f32(og, a, None, p)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    tf, tb = run(1)
+    n = int(max(1, min(32, budget_s / max(tf + tb, 1e-3))))
+    if n > 1:
+        tf, tb = run(n)
+    nvox = g.nx * g.ny * g.nz
+    gups = 2.0 * nvox * n / (tf + tb) / 1e9
+    return {"value": round(gups, 4), "unit": "GUPS", "cores": threads, "kind": "port",
+            "sample": "FP + BP of the full 512^3 volume for the first %d of 720 views "
+                      "(FP32 OpenMP restatement; fp %.2fs, bp %.2fs)" % (n, tf, tb)}
+
+
+def read_traffic(kernel, workload):
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="prec512")
+    ap.add_argument("--storage", choices=["f32", "bf16"], default="f32",
+                    help="storage type of the volume and sinogram (arithmetic is FP32)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the extra bf16-storage measurement")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="only run this many FP+BP steps (for rocprof), no JSON extras")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the path has no multi-GPU "
+              "sharding here); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from gtt import capi
+    from gtt import phantom as PH
+
+    cfg = PH.CONFIGS[args.config]
+    g = cfg.geom()
+    angles = cfg.angles()
+    nviews = len(angles)
+    nvox = g.nx * g.ny * g.nz
+    nrays = g.nu * g.nv * nviews
+    nsamp = int(np.float32(1.5) * np.float32(g.nx))
+
+    ctx = capi.Context(0)
+    ctx.set_views(angles, cfg.offsets())
+    vol = PH.voxelize(cfg.phantom(), cfg.vol_shape, cfg.vol_mm, device="cuda")
+    torch.cuda.synchronize()
+
+    def measure(storage, steps, warmup):
+        dt = torch.float32 if storage == "f32" else torch.bfloat16
+        v_in = vol.to(dt).contiguous()
+        proj = torch.empty(g.proj_shape(nviews), device="cuda", dtype=torch.float32)
+        proj_in = proj if storage == "f32" else torch.empty(proj.shape, device="cuda", dtype=dt)
+        v_out = torch.empty_like(vol)
+
+        def step():
+            ctx.fp(g, v_in, proj)
+            if storage != "f32":
+                proj_in.copy_(proj)
+            ctx.bp(g, proj_in, v_out)
+
+        for _ in range(warmup):
+            step()
+        ctx.profile(True)
+        torch.cuda.synchronize()
+        fp_ms, bp_ms = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+            f, b, _ = ctx.profile_read()
+            fp_ms.append(f)
+            bp_ms.append(b)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.profile(False)
+        return (t1 - t0) / steps * 1e3, float(np.mean(fp_ms)), float(np.mean(bp_ms)), proj, v_out
+
+    if args.profile_steps:
+        measure(args.storage, args.profile_steps, 0)
+        torch.cuda.synchronize()
+        print("profile run done", flush=True)
+        return
+
+    print("bench: %s %r, %d views, storage %s" % (cfg.name, g, nviews, args.storage),
+          file=sys.stderr, flush=True)
+    ms_step, fp_ms, bp_ms, proj, v_out = measure(args.storage, args.steps, args.warmup)
+    value = 2.0 * nvox * nviews / (ms_step / 1e3) / 1e9
+
+    s_vol = 4 if args.storage == "f32" else 2
+    kind = "fp" if fp_ms >= bp_ms else "bp"
+    kms = fp_ms if kind == "fp" else bp_ms
+    ab = algorithmic_bytes(kind, nvox, nviews, nrays, nsamp, s_vol, s_vol)
+    achieved = ab / (kms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic("k_fp_ray" if kind == "fp" else "k_bp", cfg.name),
+                "kernel": "k_fp_ray" if kind == "fp" else "k_bp",
+                "kernel_ms": round(kms, 3), "algorithmic_bytes": ab,
+                "convention": "footprint: FP 8 taps x storage bytes per sample + 4 B per ray; "
+                              "BP 4 taps x storage bytes per voxel-view + 4 B per voxel"}
+
+    extra = {"fp_ms": round(fp_ms, 3), "bp_ms": round(bp_ms, 3),
+             "gups_fp": round(nvox * nviews / (fp_ms / 1e3) / 1e9, 2),
+             "gups_bp": round(nvox * nviews / (bp_ms / 1e3) / 1e9, 2),
+             "fp_samples_per_s": round(nrays * nsamp / (fp_ms / 1e3), 1)}
+    if not args.no_variants and args.storage == "f32":
+        ms2, f2, b2, _, _ = measure("bf16", max(1, args.steps - 1), 1)
+        extra["bf16_storage"] = {"ms_per_step": round(ms2, 3), "fp_ms": round(f2, 3),
+                                 "bp_ms": round(b2, 3),
+                                 "gups": round(2.0 * nvox * nviews / (ms2 / 1e3) / 1e9, 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        vol_host = vol.cpu().numpy()
+        cpu = cpu_baseline(cfg, vol_host, angles)
+
+    line = {
+        "metric": "fwd+back GUPS at 512^3 x 720 views",
+        "value": round(value, 3),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage == "f32" else "bf16-storage/f32-arith",
+        "data": "synthetic (seeded ellipsoid phantom)",
+        "config": {"workload": "%s: %dx%dx%d volume @ %.2f mm, %d views over 360 deg, "
+                               "%dx%d detector @ %.2f mm, SAD %.0f, SDD %.0f; step = FP + BP"
+                               % (cfg.name, g.nx, g.ny, g.nz, g.lx / g.nx, nviews, g.nu, g.nv,
+                                  g.fov_u / g.nu, g.sad, g.sdd),
+                   "storage": args.storage, "samples_per_ray": nsamp},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": extra,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,296 @@
+// Voxel-driven cone-beam backprojector (optionally FDK weighted) for gfx950.
+//
+// Math: xray/conebeam_projection.cu:922-1037.  For every voxel centre and
+// view: rotate by -theta, project from the source onto the detector plane,
+// subtract the view offset, take a bilinear sample (zero outside the
+// detector) and accumulate, with the FDK weight 1/U^2 in FBP mode.
+//
+// Design
+//  * Rotation about z leaves z unchanged, so for a fixed (x, y) the rotated
+//    coordinate, the magnification SDD/(SAD+q_y), the detector u-coordinate,
+//    its interpolation weight and the FDK weight are identical for every
+//    voxel of the column.  Each thread owns KZ voxels of one column and
+//    computes those once per view; per voxel-view only the v-coordinate (one
+//    FMA), its weight, four gathers and three lerps remain.
+//  * The FDK weight uses the identity r*sin(beta-phi) = x sin(beta) - y
+//    cos(beta) = -q_y (no hypot/atan2/sin per voxel-view).
+//  * Projections are read from a zero-bordered copy (1 pixel before, 2 after
+//    in u and v) so no bounds tests are needed; the copy may be BF16.
+//  * Per-view constants (cos, sin, offsets) are wave-uniform and arrive
+//    through scalar loads.
+#include "cbct_internal.h"
+
+namespace {
+
+constexpr int BP_TX = 16, BP_TY = 16;
+
+struct BPParams {
+  int nx, ny, nz, nviews;
+  float lx, ly, lz;
+  float sad, sdd, inv_sad;
+  float su, sv;        // nu/fov_u, nv/fov_v
+  float cu, cv;        // padded texel offset: n/2 - 0.5 + 1
+  float clu, clv;      // clamp limits (= n + 1)
+  unsigned pnu;        // padded row length
+  unsigned pview;      // padded elements per view
+};
+
+template <typename T>
+__device__ __forceinline__ float ldp(const T *p, unsigned i);
+template <>
+__device__ __forceinline__ float ldp<float>(const float *p, unsigned i) {
+  return p[i];
+}
+template <>
+__device__ __forceinline__ float ldp<unsigned short>(const unsigned short *p, unsigned i) {
+  return bf16_to_f32(p[i]);
+}
+
+template <typename T, bool FBP, int KZ>
+__global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__restrict__ views,
+                                                     const T *__restrict__ ppad,
+                                                     float *__restrict__ vol, int accumulate) {
+  const int x = blockIdx.x * BP_TX + threadIdx.x;
+  const int y = blockIdx.y * BP_TY + threadIdx.y;
+  const int z0 = blockIdx.z * KZ;
+  if (x >= P.nx || y >= P.ny) return;
+  // voxel centre in mm (xray/conebeam_projection.cu:945-959)
+  const float px = ((x + 0.5f) / P.nx - 0.5f) * P.lx;
+  const float py = ((y + 0.5f) / P.ny - 0.5f) * P.ly;
+  const float dz = P.lz / P.nz;
+  const float pz0 = ((z0 + 0.5f) / P.nz - 0.5f) * P.lz;
+  float acc[KZ];
+#pragma unroll
+  for (int t = 0; t < KZ; ++t) acc[t] = 0.0f;
+
+  for (int k = 0; k < P.nviews; ++k) {
+    const ViewF vw = views[k];
+    const float qx = fmaf(vw.c, px, vw.s * py);
+    const float qy = fmaf(vw.c, py, -vw.s * px);
+    const float mag = P.sdd / (qy + P.sad);
+    float tu = fmaf(fmaf(qx, mag, -vw.off_u), P.su, P.cu);
+    tu = fminf(fmaxf(tu, 0.0f), P.clu);
+    const float flu = floorf(tu);
+    const float au = tu - flu;
+    const T *row_base = ppad + (size_t)k * P.pview + (unsigned)flu;
+    const float tv0 = fmaf(fmaf(pz0, mag, -vw.off_v), P.sv, P.cv);
+    const float dtv = dz * mag * P.sv;
+    float w = 1.0f;
+    if (FBP) {
+      const float U = (P.sad - qy) * P.inv_sad;
+      w = 1.0f / (U * U);
+    }
+#pragma unroll
+    for (int t = 0; t < KZ; ++t) {
+      float tv = fmaf((float)t, dtv, tv0);
+      tv = fminf(fmaxf(tv, 0.0f), P.clv);
+      const float flv = floorf(tv);
+      const float av = tv - flv;
+      const unsigned o = (unsigned)flv * P.pnu;
+      const float a = ldp(row_base, o), b = ldp(row_base, o + 1);
+      const float c = ldp(row_base, o + P.pnu), d = ldp(row_base, o + P.pnu + 1);
+      const float r0 = fmaf(au, b - a, a);
+      const float r1 = fmaf(au, d - c, c);
+      acc[t] = fmaf(w, fmaf(av, r1 - r0, r0), acc[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < KZ; ++t) {
+    const int z = z0 + t;
+    if (z < P.nz) {
+      const size_t idx = (size_t)x + (size_t)P.nx * ((size_t)y + (size_t)P.ny * z);
+      vol[idx] = accumulate ? vol[idx] + acc[t] : acc[t];
+    }
+  }
+}
+
+// Zero-bordered copy of the projections: (nu+3) x (nv+3) per view.
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(256) k_pad_proj(const TI *__restrict__ in, TO *__restrict__ out,
+                                                  int nu, int nv, unsigned pnu, unsigned pnv,
+                                                  size_t total);
+template <>
+__global__ void __launch_bounds__(256) k_pad_proj<float, float>(const float *__restrict__ in,
+                                                                float *__restrict__ out, int nu,
+                                                                int nv, unsigned pnu, unsigned pnv,
+                                                                size_t total) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const unsigned a = (unsigned)(t % pnu);
+    const unsigned b = (unsigned)((t / pnu) % pnv);
+    const size_t k = t / ((size_t)pnu * pnv);
+    const int u = (int)a - 1, v = (int)b - 1;
+    float val = 0.0f;
+    if (u >= 0 && u < nu && v >= 0 && v < nv) val = in[(size_t)u + (size_t)nu * ((size_t)v + (size_t)nv * k)];
+    out[t] = val;
+  }
+}
+template <>
+__global__ void __launch_bounds__(256) k_pad_proj<unsigned short, unsigned short>(
+    const unsigned short *__restrict__ in, unsigned short *__restrict__ out, int nu, int nv,
+    unsigned pnu, unsigned pnv, size_t total) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const unsigned a = (unsigned)(t % pnu);
+    const unsigned b = (unsigned)((t / pnu) % pnv);
+    const size_t k = t / ((size_t)pnu * pnv);
+    const int u = (int)a - 1, v = (int)b - 1;
+    unsigned short val = 0;
+    if (u >= 0 && u < nu && v >= 0 && v < nv) val = in[(size_t)u + (size_t)nu * ((size_t)v + (size_t)nv * k)];
+    out[t] = val;
+  }
+}
+
+// Space carver (xray/conebeam_projection.cu:822-921): nearest-texel lookups,
+// z axis flipped, count of views with 0 != p < limit; mask = count < 3.
+__global__ void __launch_bounds__(256) k_spacecarver(BPParams P, int nu, int nv,
+                                                     const ViewF *__restrict__ views,
+                                                     const float *__restrict__ proj,
+                                                     unsigned char *__restrict__ mask, float limit) {
+  const size_t nvox = (size_t)P.nx * P.ny * P.nz;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nvox;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(t % P.nx);
+    const int y = (int)((t / P.nx) % P.ny);
+    const int z = (int)(t / ((size_t)P.nx * P.ny));
+    const float px = ((x + 0.5f) / P.nx - 0.5f) * P.lx;
+    const float py = ((y + 0.5f) / P.ny - 0.5f) * P.ly;
+    const float pz = -((z + 0.5f) / P.nz - 0.5f) * P.lz;
+    int count = 0;
+    for (int k = 0; k < P.nviews; ++k) {
+      const ViewF vw = views[k];
+      const float qx = fmaf(vw.c, px, vw.s * py);
+      const float qy = fmaf(vw.c, py, -vw.s * px);
+      const float mag = P.sdd / (qy + P.sad);
+      // normalised detector coordinates, nearest texel = floor(t*N)
+      const float nu_f = (qx * mag - vw.off_u) * (P.su / nu) + 0.5f;
+      const float nv_f = (pz * mag - vw.off_v) * (P.sv / nv) + 0.5f;
+      const float fu = floorf(nu_f * nu), fv = floorf(nv_f * nv);
+      float val = 0.0f;
+      if (fu >= 0.0f && fu < (float)nu && fv >= 0.0f && fv < (float)nv)
+        val = proj[(size_t)fu + (size_t)nu * ((size_t)fv + (size_t)nv * k)];
+      if (val < limit && val != 0.0f) ++count;
+    }
+    mask[t] = count < 3 ? 1 : 0;
+  }
+}
+
+inline dim3 stride_grid(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 262144) b = 262144;
+  if (b == 0) b = 1;
+  return dim3((unsigned)b);
+}
+
+BPParams make_params(const cbct_geom *g, int nviews) {
+  BPParams P;
+  P.nx = g->nx;
+  P.ny = g->ny;
+  P.nz = g->nz;
+  P.nviews = nviews;
+  P.lx = g->lx;
+  P.ly = g->ly;
+  P.lz = g->lz;
+  P.sad = g->sad;
+  P.sdd = g->sdd;
+  P.inv_sad = 1.0f / g->sad;
+  P.su = (float)g->nu / g->fov_u;
+  P.sv = (float)g->nv / g->fov_v;
+  P.cu = 0.5f * g->nu - 0.5f + 1.0f;
+  P.cv = 0.5f * g->nv - 0.5f + 1.0f;
+  P.clu = (float)(g->nu + 1);
+  P.clv = (float)(g->nv + 1);
+  P.pnu = (unsigned)padded_dim(g->nu);
+  P.pview = (unsigned)(padded_dim(g->nu) * padded_dim(g->nv));
+  return P;
+}
+
+constexpr int KZ = 8;
+
+template <typename T>
+void launch_bp(const BPParams &P, dim3 grid, hipStream_t s, const ViewF *views, const T *pp,
+               float *vol, int fbp, int acc) {
+  dim3 block(BP_TX, BP_TY);
+  if (fbp)
+    hipLaunchKernelGGL((k_bp<T, true, KZ>), grid, block, 0, s, P, views, pp, vol, acc);
+  else
+    hipLaunchKernelGGL((k_bp<T, false, KZ>), grid, block, 0, s, P, views, pp, vol, acc);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *proj, int dtype,
+            float *vol, int fbp, int accumulate, void *stream) {
+  if (!ctx) return CBCT_ERR_ARG;
+  const char *who = "conebeam_backwards_projection";
+  if (!vol || !proj)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: illegal array pointer provided", who);
+  int rc = cbct_check_geom(ctx, g, who);
+  if (rc) return rc;
+  rc = cbct_check_views(ctx, view0, nviews, who);
+  if (rc) return rc;
+  if (dtype != CBCT_F32 && dtype != CBCT_BF16)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "Error: %s: unsupported projection dtype %d", who, dtype);
+  const size_t pview = padded_dim(g->nu) * padded_dim(g->nv);
+  rc = cbct_set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  BPParams P = make_params(g, nviews);
+  dim3 grid(ceil_div(g->nx, BP_TX), ceil_div(g->ny, BP_TY), ceil_div(g->nz, KZ));
+  if (nviews == 0) {
+    // No views: the reference clears the image unless accumulating
+    // (xray/conebeam_projection.cu:1343-1350).  A fill kernel, not a memset,
+    // so the call stays capturable in a HIP graph.
+    if (!accumulate) return cbct_fill(ctx, (size_t)g->nx * g->ny * g->nz, vol, 0.0f, stream);
+    return CBCT_OK;
+  }
+  const size_t esz = dtype == CBCT_F32 ? 4 : 2;
+  const size_t total = pview * nviews;
+  rc = cbct_reserve(ctx, ctx->proj_pad, total * esz);
+  if (rc) return rc;
+  const unsigned pnu = (unsigned)padded_dim(g->nu), pnv = (unsigned)padded_dim(g->nv);
+  if (dtype == CBCT_F32) {
+    float *pp = static_cast<float *>(ctx->proj_pad.ptr);
+    hipLaunchKernelGGL((k_pad_proj<float, float>), stride_grid(total), dim3(256), 0, s,
+                       static_cast<const float *>(proj), pp, g->nu, g->nv, pnu, pnv, total);
+    CBCT_LAUNCH_CHECK(ctx, "k_pad_proj");
+    cbct_prof_mark(ctx, PROF_BP, 0, s);
+    launch_bp<float>(P, grid, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
+    cbct_prof_mark(ctx, PROF_BP, 1, s);
+  } else {
+    unsigned short *pp = static_cast<unsigned short *>(ctx->proj_pad.ptr);
+    hipLaunchKernelGGL((k_pad_proj<unsigned short, unsigned short>), stride_grid(total), dim3(256),
+                       0, s, static_cast<const unsigned short *>(proj), pp, g->nu, g->nv, pnu, pnv,
+                       total);
+    CBCT_LAUNCH_CHECK(ctx, "k_pad_proj");
+    cbct_prof_mark(ctx, PROF_BP, 0, s);
+    launch_bp<unsigned short>(P, grid, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
+    cbct_prof_mark(ctx, PROF_BP, 1, s);
+  }
+  CBCT_LAUNCH_CHECK(ctx, "k_bp");
+  return CBCT_OK;
+}
+
+int cbct_spacecarver(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *proj,
+                     unsigned char *mask, float limit, void *stream) {
+  if (!ctx) return CBCT_ERR_ARG;
+  const char *who = "conebeam_spacecarver";
+  if (!proj || !mask)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: illegal array pointer provided", who);
+  int rc = cbct_check_geom(ctx, g, who);
+  if (rc) return rc;
+  rc = cbct_check_views(ctx, view0, nviews, who);
+  if (rc) return rc;
+  rc = cbct_set_device(ctx);
+  if (rc) return rc;
+  BPParams P = make_params(g, nviews);
+  const size_t nvox = (size_t)g->nx * g->ny * g->nz;
+  hipLaunchKernelGGL(k_spacecarver, stride_grid(nvox), dim3(256), 0, as_stream(stream), P, g->nu,
+                     g->nv, ctx->d_views_f + view0, proj, mask, limit);
+  CBCT_LAUNCH_CHECK(ctx, "k_spacecarver");
+  return CBCT_OK;
+}
+
+}  // extern "C"

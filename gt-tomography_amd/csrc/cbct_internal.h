@@ -1,0 +1,110 @@
+// Internal declarations shared by the libcbct translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "cbct.h"
+
+// Per-view projector constants.  Double precision copies feed the per-ray
+// setup of the forward projector; float copies feed the inner loops.
+struct ViewD {
+  double c, s, off_u, off_v;
+};
+struct ViewF {
+  float c, s, off_u, off_v;
+};
+
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct cbct_ctx_s {
+  int device = 0;
+  std::string err;
+  // view table
+  int nviews = 0;
+  std::vector<float> angles_deg;
+  std::vector<float> offsets;  // interleaved (u, v)
+  ViewD *d_views_d = nullptr;
+  ViewF *d_views_f = nullptr;
+  float *d_angles = nullptr;  // degrees, for the Parker weights
+  // workspaces (grow-only until trimmed)
+  DeviceBuffer vol_pad;     // padded / repacked volume for the forward projector
+  DeviceBuffer proj_pad;    // padded projections for the backprojector
+  DeviceBuffer red;         // reduction partials
+  DeviceBuffer filt_tmp;    // filter staging
+  // filter state
+  int filter_nu = 0;
+  int filter_engine = 1;
+  float *d_h = nullptr;          // fp32 spatial kernel, 2*nu entries (h[|m|] mirrored)
+  unsigned short *d_thi = nullptr; // fp16 hi Toeplitz matrix, nu x nu (padded)
+  unsigned short *d_tlo = nullptr; // fp16 lo Toeplitz matrix
+  int toe_ld = 0;                // leading dimension of the fp16 matrices
+  float toe_scale = 1.0f;        // power-of-two scale applied to the fp16 matrix
+  // cosine weights cache
+  float *d_cosw = nullptr;
+  cbct_geom cosw_geom{};
+  bool cosw_valid = false;
+  double *h_red = nullptr;       // pinned host slot for reduction results
+  // kernel timing: event pairs for the FP, BP and filter kernels
+  bool profile = false;
+  hipEvent_t ev[3][2] = {};
+  bool ev_recorded[3] = {false, false, false};
+};
+
+enum { PROF_FP = 0, PROF_BP = 1, PROF_FILTER = 2 };
+// Record the start/stop event of a profiled kernel on `s` (no-op unless enabled).
+void cbct_prof_mark(cbct_ctx ctx, int which, int stop, hipStream_t s);
+
+// Error plumbing -----------------------------------------------------------
+int cbct_fail(cbct_ctx ctx, int code, const char *fmt, ...);
+
+#define CBCT_HIP_CHECK(ctx, expr)                                                    \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess)                                                            \
+      return cbct_fail((ctx), CBCT_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+#define CBCT_LAUNCH_CHECK(ctx, what)                                                 \
+  do {                                                                               \
+    hipError_t _e = hipGetLastError();                                               \
+    if (_e != hipSuccess)                                                            \
+      return cbct_fail((ctx), CBCT_ERR_HIP, "%s launch: %s", (what), hipGetErrorString(_e)); \
+  } while (0)
+
+// Workspace growth (device current must be ctx->device).
+int cbct_reserve(cbct_ctx ctx, DeviceBuffer &b, size_t bytes);
+int cbct_set_device(cbct_ctx ctx);
+int cbct_check_geom(cbct_ctx ctx, const cbct_geom *g, const char *who);
+int cbct_check_views(cbct_ctx ctx, int view0, int nviews, const char *who);
+
+static inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// bf16 helpers --------------------------------------------------------------
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) {
+  return __uint_as_float(static_cast<unsigned int>(h) << 16);
+}
+__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
+  unsigned int u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return 0x7fc0;  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return static_cast<unsigned short>(u >> 16);
+}
+
+// Padded layouts -----------------------------------------------------------
+// The forward projector reads a zero-bordered copy of the volume: one voxel
+// of zeros before index 0 and two after index n-1 on every axis, so that a
+// clamped texel coordinate in [-1, n] never needs a bounds test.
+static inline size_t padded_dim(int n) { return static_cast<size_t>(n) + 3; }
+
+// Launch helpers
+static inline unsigned int ceil_div(size_t a, size_t b) {
+  return static_cast<unsigned int>((a + b - 1) / b);
+}

@@ -1,0 +1,387 @@
+// FDK pre-filter chain: cosine weights, Parker short-scan weights, the ramp
+// filter as a Toeplitz contraction, and half-fan offset weights.
+//
+// Reference chain (xray/conebeam_projection.cu:1483-1521): P *= C; [Parker];
+// pad each row to 2Nu (data at offset Nu/2); R2C; multiply bins 0..Nu by the
+// real response F; C2R; divide by 2Nu; crop [Nu/2, Nu/2+Nu); [offset weights].
+// Because the data occupy Nu of the 2Nu padded samples, the circular
+// convolution never wraps, and the whole FFT chain is exactly the Toeplitz
+// product  y[i] = sum_j h[|i-j|] x[j]  with
+//   h[m] = (F0 + 2 sum_{k=1}^{Nu-1} Fk cos(pi k m / Nu) + F_Nu (-1)^m) / (2Nu).
+// That is a dense (rows x Nu) * (Nu x Nu) product -- GEMM-shaped work, so it
+// runs as a tiled contraction (FP32 VALU engine, or FP16 hi/lo split MFMA
+// engine with FP32 accumulation).
+#include <cmath>
+#include <vector>
+
+#include "cbct_internal.h"
+
+namespace {
+
+// ------------------------------------------------------------------ weights
+// Parker-type short-scan weights, conebeam_projection.cu:149-194.
+__device__ __forceinline__ float parker_S(float b) {
+  if (b <= -0.5f) return 0.0f;
+  if (b < 0.5f) return 0.5f * (1.0f + sinf(3.14159265358979f * b));
+  return 1.0f;
+}
+__device__ __forceinline__ float parker_b(float alpha, float delta) {
+  return 0.1f * (2.0f * (delta - alpha) + 0.001f);
+}
+__device__ float parker_weight(int u, int nu, float beta, float delta, float tan_delta) {
+  const float eps = 0.001f;
+  const float pi = 3.14159265358979f;
+  const float alpha = -atanf(((float)u / (float)nu - 0.5f) * 2.0f * tan_delta);
+  return 0.5f * (parker_S(beta / parker_b(alpha, delta) - 0.5f) +
+                 parker_S((beta + 2.0f * (alpha - delta) - eps) / parker_b(alpha, delta) + 0.5f) -
+                 parker_S((beta - pi + 2.0f * alpha) / parker_b(-alpha, delta) - 0.5f) -
+                 parker_S((beta - pi - 2.0f * delta - eps) / parker_b(-alpha, delta) + 0.5f));
+}
+
+// out = in * C[u,v] * (parker ? w_parker : 1)
+__global__ void __launch_bounds__(256) k_cos_parker(const float *__restrict__ in,
+                                                    float *__restrict__ out,
+                                                    const float *__restrict__ cosw, int nu, int nv,
+                                                    int nviews, const float *__restrict__ angles,
+                                                    int parker, float delta, float tan_delta) {
+  const size_t plane = (size_t)nu * nv;
+  const size_t total = plane * nviews;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t k = t / plane;
+    const size_t uv = t - k * plane;
+    float v = in[t] * cosw[uv];
+    if (parker) {
+      const int u = (int)(uv % nu);
+      const float beta = angles[k] * (3.14159265358979f / 180.0f);
+      v *= parker_weight(u, nu, beta, delta, tan_delta);
+    }
+    out[t] = v;
+  }
+}
+
+// Half-fan weights: conebeam_projection.cu:233-262 (full) and :303-331 (sqrt).
+__global__ void __launch_bounds__(256) k_offset_weights(float *__restrict__ p, int nu, int nv,
+                                                        int nviews, const ViewF *__restrict__ views,
+                                                        float fu, int sqrt_variant) {
+  const size_t plane = (size_t)nu * nv;
+  const size_t total = plane * nviews;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t k = t / plane;
+    const int u = (int)((t - k * plane) % nu);
+    const float ou = views[k].off_u;
+    const float sign = ou / fabsf(ou);
+    const float omega = fu / 2.0f - fabsf(ou);
+    const float pi = 3.14159265358979f;
+    if (!sqrt_variant) {
+      const float tt = (fu * ((float)u / (float)nu - 0.5f) + ou) * sign;
+      float v = p[t];
+      if (fabsf(tt) <= omega) {
+        const float sw = sinf(pi * (tt + omega) / (4.0f * omega));
+        v *= sw * sw;
+      }
+      if (tt < -omega) v *= 0.0f;
+      p[t] = v;
+    } else {
+      const float tt = (fu * (((float)u + 0.5f) / (float)nu - 0.5f) + ou) * sign;
+      if (fabsf(tt) <= fabsf(omega)) p[t] *= sinf(pi * (tt + omega) / (4.0f * omega));
+    }
+  }
+}
+
+// ------------------------------------------------------- FP32 Toeplitz engine
+// Y[r][i] = sum_j hm[i - j + nu - 1] * X[r][j], hm = mirrored kernel (2nu-1).
+// Tile: 64 rows x 64 outputs per 256-thread block, 4x4 per thread, K-chunks
+// of 64 staged in LDS together with the 127-entry kernel window.
+constexpr int TT = 64;
+
+__global__ void __launch_bounds__(256) k_toeplitz_f32(const float *__restrict__ X,
+                                                      float *__restrict__ Y,
+                                                      const float *__restrict__ hm, int nu,
+                                                      size_t nrows) {
+  __shared__ float xs[TT][TT + 1];
+  __shared__ float hs[2 * TT];
+  const int tid = threadIdx.x;
+  const int ti = tid & 15;   // output column group
+  const int tr = tid >> 4;   // row group
+  const int i0 = blockIdx.x * TT;
+  const size_t r0 = (size_t)blockIdx.y * TT;
+  float acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+
+  for (int j0 = 0; j0 < nu; j0 += TT) {
+    __syncthreads();
+    // stage X[r0..r0+63][j0..j0+63]
+    for (int e = tid; e < TT * TT; e += 256) {
+      const int rr = e / TT, jj = e % TT;
+      const size_t r = r0 + rr;
+      const int j = j0 + jj;
+      xs[rr][jj] = (r < nrows && j < nu) ? X[r * nu + j] : 0.0f;
+    }
+    // kernel window: w = (i - i0) - (j - j0) + 63, index hm[i0 - j0 + w - 63 + nu - 1]
+    if (tid < 2 * TT - 1) {
+      const int m = i0 - j0 + tid - (TT - 1) + nu - 1;
+      hs[tid] = (m >= 0 && m < 2 * nu - 1) ? hm[m] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int jj = 0; jj < TT; ++jj) {
+      float xv[4], hv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) xv[a] = xs[tr * 4 + a][jj];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) hv[b] = hs[ti * 4 + b - jj + TT - 1];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(xv[a], hv[b], acc[a][b]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const size_t r = r0 + tr * 4 + a;
+    if (r >= nrows) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = i0 + ti * 4 + b;
+      if (i < nu) Y[r * nu + i] = acc[a][b];
+    }
+  }
+}
+
+inline dim3 stride_grid(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 262144) b = 262144;
+  if (b == 0) b = 1;
+  return dim3((unsigned)b);
+}
+
+// Host: one-sided response F[0..nu] -> spatial kernel h[0..nu-1] (FP64).
+std::vector<double> spatial_kernel(int nu, const float *F) {
+  std::vector<double> h(nu);
+  const double pi = 3.14159265358979323846;
+  const double inv = 1.0 / (2.0 * nu);
+  for (int m = 0; m < nu; ++m) {
+    double acc = (double)F[0] + (double)F[nu] * ((m & 1) ? -1.0 : 1.0);
+    for (int k = 1; k < nu; ++k) {
+      // reduce k*m mod 2nu exactly before taking the cosine
+      const long long km = ((long long)k * m) % (2LL * nu);
+      acc += 2.0 * (double)F[k] * std::cos(pi * (double)km / nu);
+    }
+    h[m] = acc * inv;
+  }
+  return h;
+}
+
+int ensure_cos_weights(cbct_ctx ctx, const cbct_geom *g) {
+  if (ctx->cosw_valid && ctx->cosw_geom.nu == g->nu && ctx->cosw_geom.nv == g->nv &&
+      ctx->cosw_geom.fov_u == g->fov_u && ctx->cosw_geom.fov_v == g->fov_v &&
+      ctx->cosw_geom.sad == g->sad && ctx->cosw_geom.sdd == g->sdd)
+    return CBCT_OK;
+  std::vector<float> w((size_t)g->nu * g->nv);
+  int rc = cbct_cosine_weights(g, w.data());
+  if (rc) return cbct_fail(ctx, rc, "compute_cosine_weights: invalid geometry");
+  if (ctx->d_cosw) (void)hipFree(ctx->d_cosw);
+  ctx->d_cosw = nullptr;
+  ctx->cosw_valid = false;
+  CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_cosw, sizeof(float) * w.size()));
+  CBCT_HIP_CHECK(ctx, hipDeviceSynchronize());
+  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_cosw, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+  ctx->cosw_geom = *g;
+  ctx->cosw_valid = true;
+  return CBCT_OK;
+}
+
+int run_toeplitz(cbct_ctx ctx, int nu, size_t nrows, const float *in, float *out, hipStream_t s) {
+  if (nrows == 0) return CBCT_OK;
+  const float *src = in;
+  if (in == out) {
+    // the tiled kernel reads whole rows while writing: stage the input
+    int rc = cbct_reserve(ctx, ctx->filt_tmp, sizeof(float) * nrows * nu);
+    if (rc) return rc;
+    CBCT_HIP_CHECK(ctx, hipMemcpyAsync(ctx->filt_tmp.ptr, in, sizeof(float) * nrows * nu,
+                                       hipMemcpyDeviceToDevice, s));
+    src = static_cast<const float *>(ctx->filt_tmp.ptr);
+  }
+  const size_t ytiles = (nrows + TT - 1) / TT;
+  if (ytiles > 0x7fffffffull)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_ramp_filter: too many rows");
+  dim3 grid(ceil_div(nu, TT), (unsigned)ytiles);
+  cbct_prof_mark(ctx, PROF_FILTER, 0, s);
+  hipLaunchKernelGGL(k_toeplitz_f32, grid, dim3(256), 0, s, src, out, ctx->d_h, nu, nrows);
+  cbct_prof_mark(ctx, PROF_FILTER, 1, s);
+  CBCT_LAUNCH_CHECK(ctx, "k_toeplitz_f32");
+  return CBCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cbct_cosine_weights(const cbct_geom *g, float *w) {
+  if (!g || !w || g->nu <= 0 || g->nv <= 0) return CBCT_ERR_ARG;
+  const double SAD = g->sad, SDD = g->sdd;
+  for (int y = 0; y < g->nv; ++y)
+    for (int x = 0; x < g->nu; ++x) {
+      const double xx = ((double)x / (double)g->nu - 0.5) * (double)g->fov_u;
+      const double yy = ((double)y / (double)g->nv - 0.5) * (double)g->fov_v;
+      const double s = SAD * xx / SDD;
+      const double v = SAD * yy / SDD;
+      w[x + (size_t)g->nu * y] = (float)(SAD / std::sqrt(SAD * SAD + s * s + v * v));
+    }
+  return CBCT_OK;
+}
+
+int cbct_default_filter(int nu, float *F) {
+  if (nu <= 0 || !F) return CBCT_ERR_ARG;
+  const int n = nu + 1;
+  const float A2 = (float)(0.2 * n * n);
+  double sum = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const float k = (float)i;
+    F[i] = (k * k < A2) ? k * A2 / (A2 - k * k) * std::exp(-A2 / (A2 - k * k)) : 0.0f;
+    sum += std::fabs((double)F[i]);
+  }
+  const float scale = (float)(n / sum);
+  for (int i = 0; i < n; ++i) F[i] *= scale;
+  return CBCT_OK;
+}
+
+int cbct_filter_set(cbct_ctx ctx, int nu, const float *F) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (nu <= 0 || nu > 8192 || !F)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "set_frequency_filter: filter must hold nu+1 taps");
+  int rc = cbct_set_device(ctx);
+  if (rc) return rc;
+  std::vector<double> h = spatial_kernel(nu, F);
+  std::vector<float> hm(2 * (size_t)nu - 1);
+  for (int d = -(nu - 1); d <= nu - 1; ++d) hm[d + nu - 1] = (float)h[d < 0 ? -d : d];
+  CBCT_HIP_CHECK(ctx, hipDeviceSynchronize());
+  if (ctx->d_h) (void)hipFree(ctx->d_h);
+  ctx->d_h = nullptr;
+  ctx->filter_nu = 0;
+  CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_h, sizeof(float) * hm.size()));
+  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_h, hm.data(), sizeof(float) * hm.size(), hipMemcpyHostToDevice));
+  ctx->filter_nu = nu;
+  return CBCT_OK;
+}
+
+int cbct_filter_engine(cbct_ctx ctx, int engine) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (engine != 0)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_filter_engine: engine %d not available", engine);
+  ctx->filter_engine = engine;
+  return CBCT_OK;
+}
+
+int cbct_ramp_filter(cbct_ctx ctx, int nu, size_t nrows, const float *in, float *out, void *stream) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (!in || !out) return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_ramp_filter: illegal array pointer provided");
+  if (ctx->filter_nu == 0)
+    return cbct_fail(ctx, CBCT_ERR_STATE, "Error: conebeam_backwards_projection: for _filtered_ backprojection both cosine weights and a filter must be provided");
+  if (nu != ctx->filter_nu)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_ramp_filter: filter was set for nu=%d, called with nu=%d", ctx->filter_nu, nu);
+  int rc = cbct_set_device(ctx);
+  if (rc) return rc;
+  return run_toeplitz(ctx, nu, nrows, in, out, as_stream(stream));
+}
+
+int cbct_fdk_prefilter(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *in,
+                       float *out, int short_scan, int offset_correct, void *stream) {
+  if (!ctx) return CBCT_ERR_ARG;
+  const char *who = "conebeam_backwards_projection";
+  if (!in || !out) return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: illegal array pointer provided", who);
+  int rc = cbct_check_geom(ctx, g, who);
+  if (rc) return rc;
+  rc = cbct_check_views(ctx, view0, nviews, who);
+  if (rc) return rc;
+  if (ctx->filter_nu == 0)
+    return cbct_fail(ctx, CBCT_ERR_STATE, "Error: %s: for _filtered_ backprojection both cosine weights and a filter must be provided", who);
+  if (ctx->filter_nu != g->nu)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: filter length %d does not match nu+1=%d", who, ctx->filter_nu + 1, g->nu + 1);
+  if (nviews == 0) return CBCT_OK;
+  rc = cbct_set_device(ctx);
+  if (rc) return rc;
+  rc = ensure_cos_weights(ctx, g);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  const size_t total = (size_t)g->nu * g->nv * nviews;
+  const float *d_angles = ctx->d_angles + view0;
+  const float delta = std::atan(g->fov_u / (2.0f * g->sdd));
+  const float tan_delta = std::tan(delta);
+  // stage 1: weights into the staging buffer
+  rc = cbct_reserve(ctx, ctx->filt_tmp, sizeof(float) * total);
+  if (rc) return rc;
+  float *tmp = static_cast<float *>(ctx->filt_tmp.ptr);
+  hipLaunchKernelGGL(k_cos_parker, stride_grid(total), dim3(256), 0, s, in, tmp, ctx->d_cosw, g->nu,
+                     g->nv, nviews, d_angles, short_scan ? 1 : 0, delta, tan_delta);
+  CBCT_LAUNCH_CHECK(ctx, "k_cos_parker");
+  // stage 2: the Toeplitz filter from the staging buffer into out
+  const size_t nrows = (size_t)g->nv * nviews;
+  dim3 grid(ceil_div(g->nu, TT), (unsigned)((nrows + TT - 1) / TT));
+  cbct_prof_mark(ctx, PROF_FILTER, 0, s);
+  hipLaunchKernelGGL(k_toeplitz_f32, grid, dim3(256), 0, s, tmp, out, ctx->d_h, g->nu, nrows);
+  cbct_prof_mark(ctx, PROF_FILTER, 1, s);
+  CBCT_LAUNCH_CHECK(ctx, "k_toeplitz_f32");
+  // stage 3: full offset weights
+  if (offset_correct) {
+    hipLaunchKernelGGL(k_offset_weights, stride_grid(total), dim3(256), 0, s, out, g->nu, g->nv,
+                       nviews, ctx->d_views_f + view0, g->fov_u, 0);
+    CBCT_LAUNCH_CHECK(ctx, "k_offset_weights");
+  }
+  return CBCT_OK;
+}
+
+int cbct_offset_weights(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, float *proj,
+                        int sqrt_variant, void *stream) {
+  if (!ctx) return CBCT_ERR_ARG;
+  const char *who = "offset_correct";
+  if (!proj) return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: illegal array pointer provided", who);
+  int rc = cbct_check_geom(ctx, g, who);
+  if (rc) return rc;
+  rc = cbct_check_views(ctx, view0, nviews, who);
+  if (rc) return rc;
+  if (nviews == 0) return CBCT_OK;
+  rc = cbct_set_device(ctx);
+  if (rc) return rc;
+  const size_t total = (size_t)g->nu * g->nv * nviews;
+  hipLaunchKernelGGL(k_offset_weights, stride_grid(total), dim3(256), 0, as_stream(stream), proj,
+                     g->nu, g->nv, nviews, ctx->d_views_f + view0, g->fov_u, sqrt_variant ? 1 : 0);
+  CBCT_LAUNCH_CHECK(ctx, "k_offset_weights");
+  return CBCT_OK;
+}
+
+int cbct_parker_weights(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, float *proj,
+                        void *stream) {
+  if (!ctx) return CBCT_ERR_ARG;
+  const char *who = "redundancy_correct";
+  if (!proj) return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: illegal array pointer provided", who);
+  int rc = cbct_check_geom(ctx, g, who);
+  if (rc) return rc;
+  rc = cbct_check_views(ctx, view0, nviews, who);
+  if (rc) return rc;
+  if (nviews == 0) return CBCT_OK;
+  rc = cbct_set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  // unit "cosine weights": reuse k_cos_parker with a ones plane
+  const size_t plane = (size_t)g->nu * g->nv;
+  rc = cbct_reserve(ctx, ctx->filt_tmp, sizeof(float) * plane);
+  if (rc) return rc;
+  float *ones = static_cast<float *>(ctx->filt_tmp.ptr);
+  rc = cbct_fill(ctx, plane, ones, 1.0f, stream);
+  if (rc) return rc;
+  const float *d_angles = ctx->d_angles + view0;
+  const float delta = std::atan(g->fov_u / (2.0f * g->sdd));
+  const size_t total = plane * nviews;
+  hipLaunchKernelGGL(k_cos_parker, stride_grid(total), dim3(256), 0, s, proj, proj, ones, g->nu,
+                     g->nv, nviews, d_angles, 1, delta, std::tan(delta));
+  CBCT_LAUNCH_CHECK(ctx, "k_cos_parker");
+  return CBCT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+/*
+ * libcbct -- MI355X (gfx950) cone-beam CT projector: the C ABI.
+ *
+ * This is the drop-in boundary for the hot path of dchansen/gt-tomography.
+ * Each entry point names the reference interface it replaces.  All arrays
+ * are first-index-fastest (Gadgetron layout):
+ *   volume       V[x + nx*(y + ny*z)]            (nx, ny, nz)
+ *   projections  P[u + nu*(v + nv*k)]            (nu, nv, nviews)
+ *
+ * Conventions
+ *  - Device pointers are caller-owned (hipMalloc / torch); host pointers are
+ *    named *_host.  The context owns its scratch (padded copies, filter
+ *    tables, reduction partials) and never allocates per call once sized.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  Work
+ *    is enqueued asynchronously unless the function returns a host value.
+ *  - Every function returns CBCT_OK (0) or a negative CBCT_ERR_* code and
+ *    never throws; cbct_last_error() returns the message.  The C++ operator
+ *    layer rethrows these as std::runtime_error with the reference's text.
+ *  - A context is single-threaded; distinct contexts may be used concurrently.
+ *    There is no global mutable state.
+ */
+#ifndef CBCT_H
+#define CBCT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cbct_ctx_s *cbct_ctx;
+
+/* Volume + detector geometry.  Mirrors CBCT_geometry (SAD, SDD, FOV) plus the
+ * operator's is_dims_in_mm (xray/CBCT_acquisition.h:24-120). */
+typedef struct {
+  int nx, ny, nz;      /* volume voxels                                   */
+  float lx, ly, lz;    /* volume extent in mm (is_dims_in_mm)            */
+  int nu, nv;          /* detector pixels                                 */
+  float fov_u, fov_v;  /* detector extent in mm (CBCT_geometry::get_FOV) */
+  float sad, sdd;      /* source-axis, source-detector distance in mm    */
+} cbct_geom;
+
+enum { CBCT_F32 = 0, CBCT_BF16 = 1 };
+
+enum {
+  CBCT_OK = 0,
+  CBCT_ERR_ARG = -1,         /* illegal pointer / size / index                  */
+  CBCT_ERR_STATE = -2,       /* e.g. no views or filter set ("setup not performed") */
+  CBCT_ERR_HIP = -3,         /* HIP runtime error                                */
+  CBCT_ERR_NOMEM = -4,       /* workspace could not be allocated                 */
+  CBCT_ERR_UNSUPPORTED = -5  /* option not available for this dtype/shape        */
+};
+
+/* ---------------------------------------------------------------- context */
+int cbct_ctx_create(int device, cbct_ctx *out);
+int cbct_ctx_destroy(cbct_ctx ctx);
+/* Message of the last failing call on ctx (ctx == NULL: last create failure). */
+const char *cbct_last_error(cbct_ctx ctx);
+/* Bytes of device scratch the context currently holds. */
+size_t cbct_workspace_bytes(cbct_ctx ctx);
+/* Kernel timing (no reference counterpart; the reference wraps whole driver
+ * phases in GPUTimer).  When enabled, the context brackets the main kernel of
+ * the next cbct_fp / cbct_bp / cbct_ramp_filter launches with HIP events on
+ * the launch stream; cbct_profile_read waits for them and returns the last
+ * durations in ms (negative if that kernel has not run since enabling). */
+int cbct_profile_enable(cbct_ctx ctx, int on);
+int cbct_profile_read(cbct_ctx ctx, double *fp_ms, double *bp_ms, double *filter_ms);
+/* Frees the context's scratch (it regrows on demand). */
+int cbct_ctx_trim(cbct_ctx ctx);
+
+/* View table: n views, angles in degrees, offsets (u, v) in mm interleaved
+ * (offsets_host may be NULL = all zero).  Replaces the per-call
+ * thrust::device_vector uploads of angles/offsets
+ * (xray/conebeam_projection.cu:600-601, :1225-1226); subsequent projector
+ * calls address views [view0, view0 + nviews) of this table. */
+int cbct_views_set(cbct_ctx ctx, const float *angles_deg_host, const float *offsets_host, int n);
+int cbct_views_count(cbct_ctx ctx);
+
+/* ------------------------------------------------------------- projectors */
+/* Forward projection, ray-driven, floor(samples_per_pixel*nx) trilinear
+ * samples per ray scaled by the step length in mm.  proj = (accumulate ? proj
+ * : 0) + A vol for views [view0, view0+nviews).  vol_dtype CBCT_F32 or
+ * CBCT_BF16 (bf16 storage, fp32 arithmetic).
+ * Replaces conebeam_forwards_projection(cuNDArray*, ...)
+ *   (xray/conebeam_projection.h:40-51; kernel xray/conebeam_projection.cu:412-528). */
+int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol,
+            int vol_dtype, float *proj, float samples_per_pixel, int accumulate, void *stream);
+
+/* Voxel-driven backprojection with bilinear detector interpolation.
+ * vol = (accumulate ? vol : 0) + sum_views w * P(project(voxel)), w = 1 or the
+ * FDK weight 1/U^2 when fbp != 0.  proj_dtype CBCT_F32 or CBCT_BF16.
+ * Replaces conebeam_backwards_projection(cuNDArray*, ...) and the
+ * conebeam_backwards_projection_kernel<FBP> template
+ *   (xray/conebeam_projection.h:57-69; kernel xray/conebeam_projection.cu:922-1037).
+ * Offset correction is a separate call (cbct_offset_weights). */
+int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *proj,
+            int proj_dtype, float *vol, int fbp, int accumulate, void *stream);
+
+/* Exact transpose of cbct_fp (ray-driven scatter, fp32 atomics):
+ * vol = (accumulate ? vol : 0) + A^T proj.  No reference counterpart: the
+ * reference (mult_M, mult_MH) pair is not adjoint; this exists for the
+ * adjoint test. */
+int cbct_fp_transpose(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *proj,
+                      float *vol, float samples_per_pixel, int accumulate, void *stream);
+
+/* Space-carving mask: mask[v] = (number of views whose nearest-texel sample
+ * p satisfies p < limit && p != 0) < 3.
+ * Replaces conebeam_spacecarver (xray/conebeam_projection.h:74-85;
+ * kernel xray/conebeam_projection.cu:822-921). */
+int cbct_spacecarver(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *proj,
+                     unsigned char *mask, float limit, void *stream);
+
+/* image *= mask.  Replaces apply_mask (xray/conebeam_projection.cu:1651-1657). */
+int cbct_apply_mask(cbct_ctx ctx, size_t n, float *image, const unsigned char *mask, void *stream);
+
+/* ----------------------------------------------------------------- FDK */
+/* Cosine weights (nv x nu, fp64 arithmetic then fp32) into a host buffer.
+ * Replaces hoCuConebeamProjectionOperator::compute_cosine_weights
+ *   (xray/hoCuConebeamProjectionOperator.cpp:64-102). */
+int cbct_cosine_weights(const cbct_geom *g, float *weights_host);
+
+/* Default (Guo et al.) frequency filter, nu+1 taps.
+ * Replaces compute_default_frequency_filter
+ *   (xray/hoCuConebeamProjectionOperator.cpp:26-62). */
+int cbct_default_filter(int nu, float *filter_host);
+
+/* Install the one-sided frequency response F[0..nu] (nu+1 floats, host).  The
+ * context converts it to the equivalent spatial Toeplitz kernel of the
+ * pad(2nu) -> R2C -> *F -> C2R -> /2nu -> crop chain, in fp64, and keeps the
+ * fp32 kernel and its fp16 hi/lo split on the device.
+ * Replaces set_frequency_filter (xray/hoCuConebeamProjectionOperator.h:117-119). */
+int cbct_filter_set(cbct_ctx ctx, int nu, const float *filter_host);
+
+/* Ramp-filter engine selection: 0 = fp32 VALU Toeplitz, 1 = fp16 hi/lo split
+ * MFMA Toeplitz with fp32 accumulation (default). */
+int cbct_filter_engine(cbct_ctx ctx, int engine);
+
+/* The FDK pre-filter chain for views [view0, view0+nviews):
+ * out = [offset_w]( filter( [parker]( in * cosine_w ) ) ).  in and out may
+ * alias.  Replaces the FBP branch of the batched backprojection
+ *   (xray/conebeam_projection.cu:1483-1521). */
+int cbct_fdk_prefilter(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *in,
+                       float *out, int short_scan, int offset_correct, void *stream);
+
+/* Row filter alone (no weights): out[r] = T in[r] for nrows rows of nu. */
+int cbct_ramp_filter(cbct_ctx ctx, int nu, size_t nrows, const float *in, float *out,
+                     void *stream);
+
+/* Half-fan offset weights in place; sqrt_variant selects the square-root form.
+ * Replaces offset_correct / offset_correct_sqrt / apply_offset_correct
+ *   (xray/conebeam_projection.cu:230-404). */
+int cbct_offset_weights(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, float *proj,
+                        int sqrt_variant, void *stream);
+
+/* Parker short-scan weights in place.  Replaces redundancy_correct
+ *   (xray/conebeam_projection.cu:141-227). */
+int cbct_parker_weights(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, float *proj,
+                        void *stream);
+
+/* ------------------------------------------------ TV / solver passes */
+/* out = (accumulate ? out : 0) + weight * D^T Pi( (weight*sigma)/(1+sigma*alpha) D x )
+ * per 3-D volume, nvol volumes back to back, periodic forward differences;
+ * one fused kernel, no 3*Nvox scratch.
+ * Replaces cuTVPrimalDualOperator<float>::primalDual
+ *   (operators/cuTVPrimalDualOperator.cu:10-106). */
+int cbct_tv_primal_dual(cbct_ctx ctx, int nx, int ny, int nz, int nvol, const float *x,
+                        float *out, float sigma, float alpha, float weight, int accumulate,
+                        void *stream);
+
+/* x = (x - tau*g + (tau/avg_lambda)*s/precon) / (1 + tau/(avg_lambda*precon)).
+ * Replaces osMOMSolverD::updateG (solvers/osMOMSolverD.h:233-247, the base
+ * class form the cuCBOS_reconstruct driver instantiates). */
+int cbct_update_g(cbct_ctx ctx, size_t n, const float *g, float *x, const float *s,
+                  const float *precon, float tau, float avg_lambda, void *stream);
+
+/* Fused TV proximal step: x_out = updateG(TV(x_in), x_in, s, precon) with the
+ * TV gradient computed in registers (single isotropic TV regulariser).
+ * x_out must not alias x_in.  Equivalent to clear(g); cbct_tv_primal_dual(x,
+ * g, ...); cbct_update_g(g, x, ...). */
+int cbct_tv_update_g(cbct_ctx ctx, int nx, int ny, int nz, int nvol, const float *x_in,
+                     float *x_out, const float *s, const float *precon, float sigma,
+                     float alpha, float weight, float tau, float avg_lambda, void *stream);
+
+/* OS-SART residual in place: r = -(r - b) * w   (r holds A_s x on entry).
+ * Replaces solvers/osSARTSolver.h:92-98. */
+int cbct_sart_residual(cbct_ctx ctx, size_t n, float *r, const float *b, const float *w,
+                       void *stream);
+
+/* OS-SART update: x = x + step * img * wx; if nonneg x = max(x, 0).
+ * Replaces solvers/osSARTSolver.h:100-105. */
+int cbct_sart_update(cbct_ctx ctx, size_t n, float *x, const float *img, const float *wx,
+                     float step, int nonneg, void *stream);
+
+/* OS-MOM gradient step: z = z + step * img * precon.
+ * Replaces solvers/osMOMSolverD.h:156-157. */
+int cbct_mom_gradient_step(cbct_ctx ctx, size_t n, float *z, const float *img,
+                           const float *precon, float step, void *stream);
+
+/* OS-MOM momentum: if nonneg x = max(x, 0); z = x*(1 + c) - c*xold with
+ * c = (told-1)/t.  Replaces solvers/osMOMSolverD.h:166-173. */
+int cbct_mom_momentum(cbct_ctx ctx, size_t n, float *x, const float *xold, float *z, float c,
+                      int nonneg, void *stream);
+
+/* Gadgetron cuNDArray element-wise primitives used by the solvers. */
+int cbct_fill(cbct_ctx ctx, size_t n, float *x, float value, void *stream);
+int cbct_scale(cbct_ctx ctx, size_t n, float *x, float a, void *stream);            /* x *= a  */
+int cbct_add_scalar(cbct_ctx ctx, size_t n, float *x, float a, void *stream);       /* x += a  */
+int cbct_axpy(cbct_ctx ctx, size_t n, float a, const float *x, float *y, void *stream); /* y += a x */
+int cbct_mul(cbct_ctx ctx, size_t n, const float *a, float *y, void *stream);       /* y *= a  */
+int cbct_div(cbct_ctx ctx, size_t n, const float *a, float *y, void *stream);       /* y /= a  */
+int cbct_sub(cbct_ctx ctx, size_t n, const float *a, float *y, void *stream);       /* y -= a  */
+int cbct_add(cbct_ctx ctx, size_t n, const float *a, float *y, void *stream);       /* y += a  */
+int cbct_clamp_min(cbct_ctx ctx, size_t n, float *x, float lo, void *stream);
+int cbct_reciprocal(cbct_ctx ctx, size_t n, float *x, void *stream);
+/* Strided broadcast multiply: y[i] *= a[i % period] (cosine weights over views). */
+int cbct_mul_periodic(cbct_ctx ctx, size_t n, const float *a, size_t period, float *y,
+                      void *stream);
+
+/* fp32 <-> bf16 storage conversion (round to nearest even). */
+int cbct_to_bf16(cbct_ctx ctx, size_t n, const float *x, unsigned short *y, void *stream);
+int cbct_from_bf16(cbct_ctx ctx, size_t n, const unsigned short *x, float *y, void *stream);
+
+/* FP64 reductions (deterministic two-pass, fp64 accumulation on the device).
+ * op: 0 = dot(a,b), 1 = nrm2(a), 2 = asum(a), 3 = sum(a).  The result is
+ * written to *out_host after the stream is synchronised.  Replaces the
+ * cuBLAS fp32 dot / nrm2 / asum the solvers and the FDK driver use. */
+enum { CBCT_DOT = 0, CBCT_NRM2 = 1, CBCT_ASUM = 2, CBCT_SUM = 3 };
+int cbct_reduce_f64(cbct_ctx ctx, int op, size_t n, const float *a, const float *b,
+                    double *out_host, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBCT_H */

@@ -1,0 +1,316 @@
+"""GPU parity of the gfx950 kernels against the FP64 oracle, through the C ABI.
+
+Tolerances (FP32 arithmetic vs FP64 oracle): projectors and filter rel-L2
+<= 1e-5; element-wise passes <= 1e-6; FP64 reductions <= 1e-12.
+BF16-storage paths are compared with the oracle run on the bf16-rounded
+input (same tolerance: the arithmetic is FP32), and their end-to-end error
+against the FP32-input oracle is bounded separately (<= 1e-3, reported).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gtt import capi
+from helpers import angles_full, bf16_round, geom, phantom_volume, rel_l2, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    return t if dtype is None else t.to(dtype)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.float().cpu().numpy().astype(np.float64)
+
+
+CASES = {
+    # name: (geometry, nviews, offsets?)
+    "cube32": (dict(nx=32, ny=32, nz=32, nu=48, nv=40), 24, False),
+    "ragged": (dict(nx=23, ny=19, nz=17, nu=37, nv=29, vol_mm=(23.0, 19.0, 25.5)), 13, True),
+    "flat": (dict(nx=40, ny=36, nz=6, nu=64, nv=12), 18, False),
+}
+
+
+def make_case(name):
+    gk, nviews, offs = CASES[name]
+    g = geom(**gk)
+    angles = angles_full(nviews, 360.0, start=3.0)
+    offsets = None
+    if offs:
+        rng = np.random.default_rng(5)
+        offsets = np.stack([rng.uniform(-2, 2, nviews), rng.uniform(-1, 1, nviews)], -1).astype(np.float32)
+    return g, angles, offsets
+
+
+def print_case(*a):
+    print("CASE", *a, flush=True)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp_parity(ctx, name):
+    g, angles, offsets = make_case(name)
+    print_case("fp", name, g)
+    vol = phantom_volume(g) + 0.001 * np.random.default_rng(1).random((g.nz, g.ny, g.nx))
+    ref = O.fp(to_oracle(g), angles, offsets, vol)
+    ctx.set_views(angles, offsets)
+    out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
+    ctx.fp(g, dev(vol), out)
+    err = rel_l2(host(out), ref)
+    print("fp rel-L2", name, err)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("fbp", [False, True])
+@pytest.mark.parametrize("name", list(CASES))
+def test_bp_parity(ctx, name, fbp):
+    g, angles, offsets = make_case(name)
+    print_case("bp", name, fbp, g)
+    proj = np.random.default_rng(2).random(g.proj_shape(len(angles)))
+    ref = O.bp(to_oracle(g), angles, offsets, proj, fbp=fbp)
+    ctx.set_views(angles, offsets)
+    out = torch.empty(g.vol_shape, device="cuda")
+    ctx.bp(g, dev(proj), out, fbp=fbp)
+    err = rel_l2(host(out), ref)
+    print("bp rel-L2", name, fbp, err)
+    assert err <= TOL
+
+
+def test_fp_bp_accumulate_and_view_ranges(ctx):
+    g, angles, offsets = make_case("cube32")
+    print_case("accumulate", g)
+    vol = phantom_volume(g)
+    og = to_oracle(g)
+    ctx.set_views(angles, offsets)
+    # FP of views [5, 15) accumulated onto a non-zero buffer
+    base = np.random.default_rng(3).random(g.proj_shape(10))
+    out = dev(base)
+    ctx.fp(g, dev(vol), out, view0=5, nviews=10, accumulate=True)
+    ref = base + O.fp(og, angles[5:15], None, vol)
+    assert rel_l2(host(out), ref) <= TOL
+    # BP of views [3, 9) accumulated
+    proj = np.random.default_rng(4).random(g.proj_shape(6))
+    vb = np.random.default_rng(5).random(g.vol_shape)
+    v = dev(vb)
+    ctx.bp(g, dev(proj), v, view0=3, nviews=6, accumulate=True)
+    ref = vb + O.bp(og, angles[3:9], None, proj)
+    assert rel_l2(host(v), ref) <= TOL
+
+
+def test_fp_bf16_storage(ctx):
+    g, angles, offsets = make_case("cube32")
+    print_case("fp-bf16", g)
+    vol = phantom_volume(g)
+    volb = bf16_round(vol).astype(np.float64)
+    ctx.set_views(angles, offsets)
+    out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
+    ctx.fp(g, dev(vol, torch.bfloat16), out)
+    got = host(out)
+    assert rel_l2(got, O.fp(to_oracle(g), angles, None, volb)) <= TOL
+    e = rel_l2(got, O.fp(to_oracle(g), angles, None, vol))
+    print("fp bf16-storage rel-L2 vs fp32-input oracle", e)
+    assert e <= 1e-3
+
+
+@pytest.mark.parametrize("fbp", [False, True])
+def test_bp_bf16_storage(ctx, fbp):
+    g, angles, offsets = make_case("ragged")
+    print_case("bp-bf16", fbp, g)
+    proj = np.random.default_rng(6).random(g.proj_shape(len(angles)))
+    projb = bf16_round(proj).astype(np.float64)
+    ctx.set_views(angles, offsets)
+    out = torch.empty(g.vol_shape, device="cuda")
+    ctx.bp(g, dev(proj, torch.bfloat16), out, fbp=fbp)
+    got = host(out)
+    assert rel_l2(got, O.bp(to_oracle(g), angles, offsets, projb, fbp=fbp)) <= TOL
+    e = rel_l2(got, O.bp(to_oracle(g), angles, offsets, proj, fbp=fbp))
+    print("bp bf16-storage rel-L2 vs fp32-input oracle", e)
+    assert e <= 1e-3
+
+
+def test_fp_transpose_parity_and_adjoint(ctx):
+    g, angles, offsets = make_case("ragged")
+    print_case("fpT", g)
+    rng = np.random.default_rng(7)
+    x = rng.random(g.vol_shape)
+    y = rng.random(g.proj_shape(len(angles)))
+    ctx.set_views(angles, offsets)
+    xd, yd = dev(x), dev(y)
+    fx = torch.zeros_like(yd)
+    ctx.fp(g, xd, fx)
+    fty = torch.zeros_like(xd)
+    ctx.fp_transpose(g, yd, fty)
+    assert rel_l2(host(fty), O.fp_transpose(to_oracle(g), angles, offsets, y)) <= TOL
+    lhs = ctx.dot(fx, yd)
+    rhs = ctx.dot(xd, fty)
+    print("adjoint", lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+
+
+@pytest.mark.parametrize("nu", [48, 37, 256])
+@pytest.mark.parametrize("which", ["ramlak", "guo"])
+def test_ramp_filter_parity(ctx, nu, which):
+    print_case("filter", nu, which)
+    F = O.ramlak_filter(nu) if which == "ramlak" else O.guo_filter(nu)
+    rows = np.random.default_rng(nu).standard_normal((70, nu))
+    ref = O.filter_rows(rows, F.astype(np.float32))
+    ctx.set_filter(F)
+    out = torch.empty((70, nu), device="cuda")
+    ctx.ramp_filter(nu, dev(rows), out)
+    err = rel_l2(host(out), ref)
+    print("filter rel-L2", nu, which, err)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("short_scan,offset", [(False, False), (True, False), (False, True)])
+def test_fdk_prefilter_parity(ctx, short_scan, offset):
+    g = geom(16, 16, 16, 40, 24, fov=(60.0, 36.0))
+    n = 20
+    arc = 200.0 if short_scan else 360.0
+    angles = angles_full(n, arc)
+    offsets = np.zeros((n, 2), np.float32)
+    if offset:
+        offsets[:, 0] = 9.0
+    print_case("prefilter", short_scan, offset)
+    proj = np.random.default_rng(8).random(g.proj_shape(n))
+    F = O.ramlak_filter(g.nu)
+    ref = O.fdk_prefilter(to_oracle(g), angles, offsets, proj, F.astype(np.float32),
+                          short_scan=short_scan, offset_correct=offset)
+    ctx.set_views(angles, offsets)
+    ctx.set_filter(F)
+    pin = dev(proj)
+    out = torch.empty_like(pin)
+    ctx.fdk_prefilter(g, pin, out, short_scan=short_scan, offset_correct=offset)
+    err = rel_l2(host(out), ref)
+    print("prefilter rel-L2", err)
+    assert err <= TOL
+    # in place
+    ctx.fdk_prefilter(g, pin, pin, short_scan=short_scan, offset_correct=offset)
+    assert rel_l2(host(pin), ref) <= TOL
+
+
+@pytest.mark.parametrize("sqrt_variant", [False, True])
+def test_offset_and_parker_weights(ctx, sqrt_variant):
+    g = geom(8, 8, 8, 40, 6, fov=(40.0, 6.0))
+    angles = angles_full(10, 220.0)
+    offsets = np.stack([np.linspace(4, 8, 10), np.zeros(10)], -1).astype(np.float32)
+    print_case("weights", sqrt_variant)
+    ctx.set_views(angles, offsets)
+    p = np.random.default_rng(9).random(g.proj_shape(10))
+    d = dev(p)
+    ctx.offset_weights(g, d, sqrt_variant=sqrt_variant)
+    ref = O.offset_weights(to_oracle(g), offsets, p.copy(), sqrt_variant=sqrt_variant)
+    assert rel_l2(host(d), ref) <= 1e-6
+    d = dev(p)
+    ctx.parker_weights(g, d)
+    ref = O.parker_weights(to_oracle(g), angles, p.copy())
+    assert rel_l2(host(d), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(6, 7, 9), (2, 5, 6, 8)])
+def test_tv_parity_and_fused_update(ctx, shape):
+    print_case("tv", shape)
+    rng = np.random.default_rng(10)
+    x = rng.random(shape)
+    out0 = rng.random(shape)
+    sigma, alpha, w = 3.0, 0.0, 1.0
+    ref = out0.copy()
+    O.tv_primal_dual(x, ref, sigma, alpha, w, accumulate=True)
+    xd = dev(x)
+    od = dev(out0)
+    ctx.tv_primal_dual(xd, od, sigma, alpha, w, accumulate=True)
+    assert rel_l2(host(od), ref) <= 1e-6
+    # fused TV + updateG equals clear + TV + updateG
+    s, P = rng.random(shape), rng.random(shape) + 0.5
+    tau, lam = 0.05, 1e-3
+    g = torch.zeros_like(xd)
+    ctx.tv_primal_dual(xd, g, sigma, alpha, w, accumulate=True)
+    x_unf = xd.clone()
+    ctx.update_g(g, x_unf, dev(s), dev(P), tau, lam)
+    x_f = torch.empty_like(xd)
+    ctx.tv_update_g(xd, x_f, dev(s), dev(P), sigma, alpha, w, tau, lam)
+    assert torch.equal(x_unf, x_f)
+    gref = np.zeros(shape)
+    O.tv_primal_dual(x, gref, sigma, alpha, w)
+    assert rel_l2(host(x_f), O.update_g(gref, x, s, P, tau, lam)) <= 1e-6
+
+
+def test_elementwise_and_reductions(ctx):
+    print_case("elementwise")
+    rng = np.random.default_rng(11)
+    n = 100003
+    a, b, c = rng.random(n) + 0.1, rng.random(n) + 0.1, rng.random(n)
+    A, B, C = dev(a), dev(b), dev(c)
+    ctx.ew("mul", A, B)
+    ctx.ew("axpy", -2.0, C, B)
+    ctx.ew("clamp_min", B, 0.05)
+    ctx.ew("reciprocal", B)
+    exp = 1.0 / np.maximum(b * a - 2.0 * c, 0.05)
+    assert rel_l2(host(B), exp) <= 1e-6
+    x = rng.standard_normal(n)
+    X = dev(x)
+    xf = x.astype(np.float32).astype(np.float64)
+    assert ctx.asum(X) == pytest.approx(np.abs(xf).sum(), rel=1e-12)
+    assert ctx.nrm2(X) == pytest.approx(np.sqrt((xf * xf).sum()), rel=1e-12)
+    yf = b.astype(np.float32).astype(np.float64)
+    assert ctx.dot(X, dev(b)) == pytest.approx(np.dot(xf, yf), rel=1e-12)
+    # fused solver passes
+    r, bb, w = rng.random(n), rng.random(n), rng.random(n)
+    R = dev(r)
+    ctx.sart_residual(R, dev(bb), dev(w))
+    assert rel_l2(host(R), -(r - bb) * w) <= 1e-6
+    xs, img, wx = rng.standard_normal(n), rng.random(n), rng.random(n)
+    XS = dev(xs)
+    ctx.sart_update(XS, dev(img), dev(wx), 0.7, True)
+    assert rel_l2(host(XS), np.maximum(xs + 0.7 * img * wx, 0.0)) <= 1e-6
+    z, xo = rng.random(n), rng.random(n)
+    xx = rng.standard_normal(n)
+    XX, Z = dev(xx), dev(z)
+    ctx.mom_momentum(XX, dev(xo), Z, 0.4, True)
+    xc = np.maximum(xx, 0)
+    assert rel_l2(host(Z), xc * 1.4 - 0.4 * xo) <= 1e-6
+    assert rel_l2(host(XX), xc) == 0.0
+
+
+def test_bf16_conversion(ctx):
+    x = np.random.default_rng(12).standard_normal(5000).astype(np.float32)
+    X = dev(x)
+    H = torch.empty(5000, dtype=torch.bfloat16, device="cuda")
+    capi.load().cbct_to_bf16(ctx.h, 5000, capi._ptr(X), capi._ptr(H), capi._stream())
+    assert np.array_equal(host(H).astype(np.float32), bf16_round(x))
+    assert torch.equal(H, X.to(torch.bfloat16))
+
+
+def test_errors_are_reported(ctx):
+    g = geom(8, 8, 8, 10, 10)
+    ctx.set_views(angles_full(4))
+    out = torch.zeros((4, 10, 10), device="cuda")
+    vol = torch.zeros((8, 8, 8), device="cuda")
+    with pytest.raises(capi.CbctError, match="inconsistent sizes"):
+        ctx.fp(g, vol, out, view0=2, nviews=4)
+    bad = geom(8, 8, 8, 10, 10)
+    bad.nx = 0
+    with pytest.raises(capi.CbctError, match="dimensions"):
+        ctx.fp(bad, vol, out)
+    c2 = capi.Context(0)
+    with pytest.raises(capi.CbctError, match="setup not performed"):
+        c2.bp(g, out, vol)
+    with pytest.raises(capi.CbctError, match="filter must be provided"):
+        c2.set_views(angles_full(4))
+        c2.fdk_prefilter(g, out, out)
+    c2.close()
