@@ -160,8 +160,8 @@ def main():
     achieved = ab / (kms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic("k_fp_ray" if kind == "fp" else "k_bp", cfg.name),
-                "kernel": "k_fp_ray" if kind == "fp" else "k_bp",
+                "traffic": read_traffic("k_fp_col" if kind == "fp" else "k_bp", cfg.name),
+                "kernel": "k_fp_col" if kind == "fp" else "k_bp",
                 "kernel_ms": round(kms, 3), "algorithmic_bytes": ab,
                 "convention": "footprint: FP 8 taps x storage bytes per sample + 4 B per ray; "
                               "BP 4 taps x storage bytes per voxel-view + 4 B per voxel"}

@@ -41,7 +41,8 @@ struct cbct_ctx_s {
   DeviceBuffer filt_tmp;    // filter staging
   // filter state
   int filter_nu = 0;
-  int filter_engine = 1;
+  int filter_engine = 0;
+  int fp_engine = 1;             // 1: detector-column sharing, 0: one thread per ray
   float *d_h = nullptr;          // fp32 spatial kernel, 2*nu entries (h[|m|] mirrored)
   unsigned short *d_thi = nullptr; // fp16 hi Toeplitz matrix, nu x nu (padded)
   unsigned short *d_tlo = nullptr; // fp16 lo Toeplitz matrix

@@ -5,21 +5,34 @@
 // n = int(samples_per_pixel*nx) left-Riemann trilinear samples, result scaled
 // by the step length in mm).
 //
-// Design
+// Common design
 //  * The per-ray setup (rotation, slab clipping, step length, start point and
 //    increment in texel space) runs in FP64, once per ray.  Only the sample
 //    loop is FP32.  Each sample position is p0 + m*q with one FMA per axis, so
 //    coordinate error does not grow along the ray.
-//  * Interpolation is done by hand (no texture unit): the volume is first
-//    copied into a zero-bordered layout (1 voxel before, 2 after on every
-//    axis) so the inner loop needs no bounds tests -- coordinates are
+//  * Interpolation is done by hand (no texture unit) from a zero-bordered copy
+//    of the volume (1 voxel before, 2 after on every axis): coordinates are
 //    clamped to [-1, n] and the border supplies the zeros of the reference's
-//    border addressing mode.
-//  * Storage may be FP32 or BF16 (bf16 halves the gather bytes; arithmetic is
-//    FP32 either way).
-//  * One thread per ray, 16x16 detector tiles per workgroup (a wave covers a
-//    16x4 pixel tile), so the 64 samples a wave issues at step m are close
-//    together in the volume and the gathers share cache lines.
+//    border addressing mode, so the inner loops carry no bounds tests.
+//  * Storage may be FP32 or BF16 (arithmetic is FP32 either way).
+//
+// Engine 1 (default): detector-column sharing.
+//  All rays of one detector column (one view, one u) lie in the vertical plane
+//  through the source.  Their x/y slab limits are identical, and the z slab
+//  never raises a1 (its lower limit is <= 0), so unless a ray leaves through
+//  the top or bottom face its (a1, a2) -- and therefore its x/y sample
+//  positions -- equal the column's at every step m.  One wave owns a column:
+//  per step it bilinearly interpolates the z-line at the shared (x, y) from a
+//  z-fastest copy of the volume (coalesced loads, 64 consecutive z per
+//  instruction) into LDS, and every ray then does one z-lerp from LDS.  The
+//  arithmetic order (x-lerps, y-lerp, z-lerp) and sample coordinates are those
+//  of engine 0, so both engines produce bitwise-identical projections.  Rays
+//  clipped by the z slab run the per-ray loop.  Blocks are remapped so each
+//  XCD walks a contiguous run of columns (neighbouring columns share z-lines
+//  in that XCD's L2).
+//
+// Engine 0: one thread per ray, 16x16 detector tiles, x-fastest padded
+// volume, 8 gathers per sample.  Also the basis of the exact transpose.
 #include "cbct_internal.h"
 
 namespace {
@@ -27,7 +40,8 @@ namespace {
 struct FPParams {
   int nu, nv, n;
   int nx, ny, nz;
-  unsigned px, pxy;           // padded strides (elements)
+  unsigned px, pxy;           // x-fastest padded strides (elements)
+  unsigned pz, pzx;           // z-fastest padded strides: z + pz*(x + px*y)
   double lx, ly, lz;
   double fu, fv, sad, sdd;
   float cx, cy, cz;           // clamp limits in padded texel space (= n_axis + 1)
@@ -44,43 +58,64 @@ __device__ __forceinline__ float ld<unsigned short>(const unsigned short *p, uns
   return bf16_to_f32(p[i]);
 }
 
-// Per-ray setup in FP64.  Returns the padded-texel start (p) and per-sample
-// increment (q) and the step length in mm.
-__device__ __forceinline__ void ray_setup(const FPParams &P, const ViewD &vw, int i, int j,
-                                          float p[3], float q[3], float *step) {
+// Column part of the ray setup (shared by every ray with detector column i).
+struct ColSetup {
+  double S0, S1, d0, d1, lo, hi;  // lo/hi: slab limits from the x and y faces
+};
+
+__device__ __forceinline__ ColSetup col_setup(const FPParams &P, const ViewD &vw, int i) {
+  ColSetup C;
   const double c = vw.c, s = vw.s;
-  const double S0 = s * P.sad, S1 = -c * P.sad;  // R(theta) (0,-SAD,0)
+  C.S0 = s * P.sad;  // R(theta) (0,-SAD,0)
+  C.S1 = -c * P.sad;
   const double pu = ((i + 0.5) / P.nu - 0.5) * P.fu + vw.off_u;
-  const double pv = ((j + 0.5) / P.nv - 0.5) * P.fv + vw.off_v;
   const double add = P.sdd - P.sad;
-  const double d0 = (c * pu - s * add) - S0;
-  const double d1 = (s * pu + c * add) - S1;
-  const double d2 = pv;
-  const double hx = P.lx * 0.5, hy = P.ly * 0.5, hz = P.lz * 0.5;
-  double tA = (hx - S0) / d0, tB = (-hx - S0) / d0;
-  double lo = fmin(tB, tA), hi = fmax(tB, tA);
-  tA = (hy - S1) / d1;
-  tB = (-hy - S1) / d1;
-  lo = fmax(lo, fmin(tB, tA));
-  hi = fmin(hi, fmax(tB, tA));
-  tA = hz / d2;
-  tB = -hz / d2;
-  lo = fmax(lo, fmin(tB, tA));
-  hi = fmin(hi, fmax(tB, tA));
-  const double a1 = fmax(lo, 0.0);
-  const double a2 = fmin(hi, 1.0);
-  const double span = a2 - a1;
-  const double h = span / P.n;
-  *step = (float)(sqrt(d0 * d0 + d1 * d1 + d2 * d2) * fabs(span) / P.n);
-  // texel = normalised*N - 0.5, plus 1 for the leading zero border
-  p[0] = (float)(((S0 + a1 * d0) / P.lx + 0.5) * P.nx + 0.5);
-  p[1] = (float)(((S1 + a1 * d1) / P.ly + 0.5) * P.ny + 0.5);
-  p[2] = (float)(((a1 * d2) / P.lz + 0.5) * P.nz + 0.5);
-  q[0] = (float)(d0 / P.lx * h * P.nx);
-  q[1] = (float)(d1 / P.ly * h * P.ny);
-  q[2] = (float)(d2 / P.lz * h * P.nz);
+  C.d0 = (c * pu - s * add) - C.S0;
+  C.d1 = (s * pu + c * add) - C.S1;
+  const double hx = P.lx * 0.5, hy = P.ly * 0.5;
+  double tA = (hx - C.S0) / C.d0, tB = (-hx - C.S0) / C.d0;
+  C.lo = fmin(tB, tA);
+  C.hi = fmax(tB, tA);
+  tA = (hy - C.S1) / C.d1;
+  tB = (-hy - C.S1) / C.d1;
+  C.lo = fmax(C.lo, fmin(tB, tA));
+  C.hi = fmin(C.hi, fmax(tB, tA));
+  return C;
 }
 
+struct RaySetup {
+  double a1, a2;
+  float p[3], q[3], step;
+};
+
+// Per-ray completion of the setup (row j).
+__device__ __forceinline__ RaySetup ray_finish(const FPParams &P, const ViewD &vw,
+                                               const ColSetup &C, int j) {
+  RaySetup R;
+  const double pv = ((j + 0.5) / P.nv - 0.5) * P.fv + vw.off_v;
+  const double d2 = pv;
+  const double hz = P.lz * 0.5;
+  const double tA = hz / d2, tB = -hz / d2;
+  const double lo = fmax(C.lo, fmin(tB, tA));
+  const double hi = fmin(C.hi, fmax(tB, tA));
+  R.a1 = fmax(lo, 0.0);
+  R.a2 = fmin(hi, 1.0);
+  const double span = R.a2 - R.a1;
+  const double h = span / P.n;
+  R.step = (float)(sqrt(C.d0 * C.d0 + C.d1 * C.d1 + d2 * d2) * fabs(span) / P.n);
+  // texel = normalised*N - 0.5, plus 1 for the leading zero border
+  R.p[0] = (float)(((C.S0 + R.a1 * C.d0) / P.lx + 0.5) * P.nx + 0.5);
+  R.p[1] = (float)(((C.S1 + R.a1 * C.d1) / P.ly + 0.5) * P.ny + 0.5);
+  R.p[2] = (float)(((R.a1 * d2) / P.lz + 0.5) * P.nz + 0.5);
+  R.q[0] = (float)(C.d0 / P.lx * h * P.nx);
+  R.q[1] = (float)(C.d1 / P.ly * h * P.ny);
+  R.q[2] = (float)(d2 / P.lz * h * P.nz);
+  return R;
+}
+
+__device__ __forceinline__ float clampf(float v, float hi) { return fminf(fmaxf(v, 0.0f), hi); }
+
+// ------------------------------------------------------------------ engine 0
 template <typename T>
 __global__ void __launch_bounds__(256) k_fp_ray(FPParams P, const ViewD *__restrict__ views,
                                                 const T *__restrict__ vol, float *__restrict__ proj,
@@ -90,15 +125,14 @@ __global__ void __launch_bounds__(256) k_fp_ray(FPParams P, const ViewD *__restr
   const int k = blockIdx.z;
   if (i >= P.nu || j >= P.nv) return;
   const ViewD vw = views[k];
-  float p[3], q[3], step;
-  ray_setup(P, vw, i, j, p, q, &step);
+  const RaySetup R = ray_finish(P, vw, col_setup(P, vw, i), j);
   float acc = 0.0f;
   const unsigned px = P.px, pxy = P.pxy;
   for (int m = 0; m < P.n; ++m) {
     const float fm = (float)m;
-    float sx = fminf(fmaxf(fmaf(fm, q[0], p[0]), 0.0f), P.cx);
-    float sy = fminf(fmaxf(fmaf(fm, q[1], p[1]), 0.0f), P.cy);
-    float sz = fminf(fmaxf(fmaf(fm, q[2], p[2]), 0.0f), P.cz);
+    const float sx = clampf(fmaf(fm, R.q[0], R.p[0]), P.cx);
+    const float sy = clampf(fmaf(fm, R.q[1], R.p[1]), P.cy);
+    const float sz = clampf(fmaf(fm, R.q[2], R.p[2]), P.cz);
     const float flx = floorf(sx), fly = floorf(sy), flz = floorf(sz);
     const float ax = sx - flx, ay = sy - fly, az = sz - flz;
     const unsigned base = (unsigned)flx + px * (unsigned)fly + pxy * (unsigned)flz;
@@ -115,7 +149,7 @@ __global__ void __launch_bounds__(256) k_fp_ray(FPParams P, const ViewD *__restr
     acc += fmaf(az, c1 - c0, c0);
   }
   const size_t idx = (size_t)i + (size_t)P.nu * ((size_t)j + (size_t)P.nv * k);
-  const float r = acc * step;
+  const float r = acc * R.step;
   proj[idx] = accumulate ? proj[idx] + r : r;
 }
 
@@ -128,16 +162,15 @@ __global__ void __launch_bounds__(256) k_fp_ray_transpose(FPParams P, const View
   const int k = blockIdx.z;
   if (i >= P.nu || j >= P.nv) return;
   const ViewD vw = views[k];
-  float p[3], q[3], step;
-  ray_setup(P, vw, i, j, p, q, &step);
-  const float y = proj[(size_t)i + (size_t)P.nu * ((size_t)j + (size_t)P.nv * k)] * step;
+  const RaySetup R = ray_finish(P, vw, col_setup(P, vw, i), j);
+  const float y = proj[(size_t)i + (size_t)P.nu * ((size_t)j + (size_t)P.nv * k)] * R.step;
   if (y == 0.0f) return;
   const unsigned px = P.px, pxy = P.pxy;
   for (int m = 0; m < P.n; ++m) {
     const float fm = (float)m;
-    float sx = fminf(fmaxf(fmaf(fm, q[0], p[0]), 0.0f), P.cx);
-    float sy = fminf(fmaxf(fmaf(fm, q[1], p[1]), 0.0f), P.cy);
-    float sz = fminf(fmaxf(fmaf(fm, q[2], p[2]), 0.0f), P.cz);
+    const float sx = clampf(fmaf(fm, R.q[0], R.p[0]), P.cx);
+    const float sy = clampf(fmaf(fm, R.q[1], R.p[1]), P.cy);
+    const float sz = clampf(fmaf(fm, R.q[2], R.p[2]), P.cz);
     const float flx = floorf(sx), fly = floorf(sy), flz = floorf(sz);
     const float ax = sx - flx, ay = sy - fly, az = sz - flz;
     const unsigned base = (unsigned)flx + px * (unsigned)fly + pxy * (unsigned)flz;
@@ -153,7 +186,169 @@ __global__ void __launch_bounds__(256) k_fp_ray_transpose(FPParams P, const View
   }
 }
 
-// Copy a volume into the zero-bordered layout.
+// ------------------------------------------------------------------ engine 1
+// Map the linear block id so that XCD x (which receives blocks b == x mod 8)
+// processes the contiguous logical range [start_x, start_x + count_x).
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
+  const unsigned q = nblocks >> 3, rem = nblocks & 7u;
+  const unsigned x = b & 7u, idx = b >> 3;
+  return x * q + (x < rem ? x : rem) + idx;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One 64-lane workgroup per (detector column, row group, view).  RPL rays per
+// lane: rows j = row0 + r*64 + lane.
+template <typename T, int RPL>
+__global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restrict__ views,
+                                               const T *__restrict__ vt, float *__restrict__ proj,
+                                               int accumulate, unsigned ngroups, unsigned nblocks) {
+  extern __shared__ float zl[];  // pz floats: one interpolated z-line
+  const unsigned L = xcd_remap(blockIdx.x, nblocks);
+  const int i = (int)(L % (unsigned)P.nu);
+  const unsigned rest = L / (unsigned)P.nu;
+  const int grp = (int)(rest % ngroups);
+  const int k = (int)(rest / ngroups);
+  const int lane = threadIdx.x;
+  const ViewD vw = views[k];
+  const ColSetup C = col_setup(P, vw, i);
+  const double a1c = fmax(C.lo, 0.0);
+  const double a2c = fmin(C.hi, 1.0);
+  const bool col_ok = (a2c - a1c) >= 0.0;   // h >= 0 keeps tz monotone in the row
+
+  const int row0 = grp * 64 * RPL;
+  float pz[RPL], qz[RPL], acc[RPL], stp[RPL];
+  unsigned shared_mask = 0, valid_mask = 0;
+  int jmin = 0x7fffffff, jmax = -1;
+  RaySetup Rc;  // the column's x/y coordinates (from any shared ray)
+  Rc.p[0] = Rc.p[1] = Rc.q[0] = Rc.q[1] = 0.0f;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int j = row0 + r * 64 + lane;
+    acc[r] = 0.0f;
+    pz[r] = qz[r] = stp[r] = 0.0f;
+    if (j < P.nv) {
+      valid_mask |= 1u << r;
+      const RaySetup R = ray_finish(P, vw, C, j);
+      stp[r] = R.step;
+      pz[r] = R.p[2];
+      qz[r] = R.q[2];
+      if (col_ok && R.a1 == a1c && R.a2 == a2c) {
+        shared_mask |= 1u << r;
+        jmin = min(jmin, j);
+        jmax = max(jmax, j);
+        Rc = R;
+      }
+    }
+  }
+  const int gjmin = wave_min_i(jmin);
+  const int gjmax = wave_max_i(jmax);
+
+  if (gjmax >= 0) {
+    // x/y trace and the z range from the first/last shared rays (identical
+    // FP64 code, so identical floats to the per-ray values)
+    const RaySetup Rlo = ray_finish(P, vw, C, gjmin);
+    const RaySetup Rhi = ray_finish(P, vw, C, gjmax);
+    const float px0 = Rlo.p[0], qx = Rlo.q[0], py0 = Rlo.p[1], qy = Rlo.q[1];
+    const float pzl = Rlo.p[2], qzl = Rlo.q[2], pzh = Rhi.p[2], qzh = Rhi.q[2];
+    // rays outside the shared set read harmless in-range LDS entries
+#pragma unroll
+    for (int r = 0; r < RPL; ++r)
+      if (!(shared_mask & (1u << r))) {
+        pz[r] = pzl;
+        qz[r] = qzl;
+      }
+    const unsigned PZ = P.pz, PZX = P.pzx;
+    for (int m = 0; m < P.n; ++m) {
+      const float fm = (float)m;
+      const float sx = clampf(fmaf(fm, qx, px0), P.cx);
+      const float sy = clampf(fmaf(fm, qy, py0), P.cy);
+      const float flx = floorf(sx), fly = floorf(sy);
+      const float ax = sx - flx, ay = sy - fly;
+      const int zlo = (int)floorf(clampf(fmaf(fm, qzl, pzl), P.cz));
+      const int zhi = (int)floorf(clampf(fmaf(fm, qzh, pzh), P.cz)) + 1;
+      const unsigned base = ((unsigned)fly * P.px + (unsigned)flx) * PZ;
+      for (int z = zlo + lane; z <= zhi; z += 64) {
+        const float v00 = ld(vt, base + z), v10 = ld(vt, base + PZ + z);
+        const float v01 = ld(vt, base + PZX + z), v11 = ld(vt, base + PZX + PZ + z);
+        const float c0 = fmaf(ax, v10 - v00, v00);
+        const float c1 = fmaf(ax, v11 - v01, v01);
+        zl[z - zlo] = fmaf(ay, c1 - c0, c0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+        const float tz = clampf(fmaf(fm, qz[r], pz[r]), P.cz);
+        const float flz = floorf(tz);
+        const float az = tz - flz;
+        const int loc = (int)flz - zlo;
+        const float a = zl[loc], b = zl[loc + 1];
+        acc[r] += fmaf(az, b - a, a);
+      }
+      __syncthreads();
+    }
+  }
+
+  // Rays clipped by the z faces (or a column that misses the box): per-ray
+  // loop on the z-fastest layout, same arithmetic order as engine 0.
+  const unsigned todo = valid_mask & ~shared_mask;
+  if (todo) {
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      if (!(todo & (1u << r))) continue;
+      const int j = row0 + r * 64 + lane;
+      const RaySetup R = ray_finish(P, vw, C, j);
+      float a = 0.0f;
+      for (int m = 0; m < P.n; ++m) {
+        const float fm = (float)m;
+        const float sx = clampf(fmaf(fm, R.q[0], R.p[0]), P.cx);
+        const float sy = clampf(fmaf(fm, R.q[1], R.p[1]), P.cy);
+        const float sz = clampf(fmaf(fm, R.q[2], R.p[2]), P.cz);
+        const float flx = floorf(sx), fly = floorf(sy), flz = floorf(sz);
+        const float ax = sx - flx, ay = sy - fly, az = sz - flz;
+        const unsigned base = ((unsigned)fly * P.px + (unsigned)flx) * P.pz + (unsigned)flz;
+        const float v000 = ld(vt, base), v100 = ld(vt, base + P.pz);
+        const float v010 = ld(vt, base + P.pzx), v110 = ld(vt, base + P.pzx + P.pz);
+        const float v001 = ld(vt, base + 1), v101 = ld(vt, base + P.pz + 1);
+        const float v011 = ld(vt, base + P.pzx + 1), v111 = ld(vt, base + P.pzx + P.pz + 1);
+        const float c00 = fmaf(ax, v100 - v000, v000);
+        const float c10 = fmaf(ax, v110 - v010, v010);
+        const float c01 = fmaf(ax, v101 - v001, v001);
+        const float c11 = fmaf(ax, v111 - v011, v011);
+        const float c0 = fmaf(ay, c10 - c00, c00);
+        const float c1 = fmaf(ay, c11 - c01, c01);
+        a += fmaf(az, c1 - c0, c0);
+      }
+      acc[r] = a;
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    if (!(valid_mask & (1u << r))) continue;
+    const int j = row0 + r * 64 + lane;
+    const size_t idx = (size_t)i + (size_t)P.nu * ((size_t)j + (size_t)P.nv * k);
+    const float v = acc[r] * stp[r];
+    proj[idx] = accumulate ? proj[idx] + v : v;
+  }
+}
+
+// ------------------------------------------------------------ layouts
+// Copy a volume into the x-fastest zero-bordered layout.
 template <typename T>
 __global__ void __launch_bounds__(256) k_pad_volume(const T *__restrict__ vol, T *__restrict__ out,
                                                     int nx, int ny, int nz, unsigned px,
@@ -169,6 +364,36 @@ __global__ void __launch_bounds__(256) k_pad_volume(const T *__restrict__ vol, T
     if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
       v = vol[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
     out[t] = v;
+  }
+}
+
+// Transposing copy into the z-fastest zero-bordered layout
+// out[z' + pz*(x' + px*y')] = vol[x'-1, y'-1, z'-1] (0 outside).
+// 32x32 (x', z') tiles through LDS so both the reads (along x) and the writes
+// (along z) are coalesced.  grid = (ceil(pz/32), ceil(px/32), py), block 32x8.
+template <typename T>
+__global__ void __launch_bounds__(256) k_pad_transpose(const T *__restrict__ vol, T *__restrict__ out,
+                                                       int nx, int ny, int nz, unsigned px,
+                                                       unsigned pz) {
+  __shared__ T tile[32][33];
+  const int z0 = blockIdx.x * 32, x0 = blockIdx.y * 32, yp = blockIdx.z;
+  const int y = yp - 1;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  // read: rows of x for 32 z
+  for (int r = ty; r < 32; r += 8) {
+    const int zp = z0 + r, xp = x0 + tx;
+    const int x = xp - 1, z = zp - 1;
+    T v = T(0);
+    if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+      v = vol[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  // write: columns of z for 32 x
+  for (int r = ty; r < 32; r += 8) {
+    const int xp = x0 + r, zp = z0 + tx;
+    if (xp < (int)px && zp < (int)pz)
+      out[(size_t)zp + (size_t)pz * ((size_t)xp + (size_t)px * yp)] = tile[tx][r];
   }
 }
 
@@ -200,6 +425,9 @@ inline dim3 stride_grid(size_t n) {
   return dim3((unsigned)b);
 }
 
+// z-fastest padded length: nz + 3 rounded up to 16 elements
+inline size_t zpadded(int nz) { return (padded_dim(nz) + 15) & ~size_t(15); }
+
 int make_params(cbct_ctx ctx, const cbct_geom *g, float spp, FPParams &P) {
   const float fn = spp * (float)g->nx;
   if (!(fn >= 1.0f) || fn > 1e6f)
@@ -211,10 +439,14 @@ int make_params(cbct_ctx ctx, const cbct_geom *g, float spp, FPParams &P) {
   P.ny = g->ny;
   P.nz = g->nz;
   const size_t px = padded_dim(g->nx), py = padded_dim(g->ny), pz = padded_dim(g->nz);
-  if (px * py * pz >= (1ull << 32) - 1)
+  const size_t pzz = zpadded(g->nz);
+  if (px * py * pzz >= (1ull << 32) - 1)
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "Error: conebeam_forwards_projection: volume too large for 32-bit indexing");
+  (void)pz;
   P.px = (unsigned)px;
   P.pxy = (unsigned)(px * py);
+  P.pz = (unsigned)pzz;
+  P.pzx = (unsigned)(pzz * px);
   P.lx = g->lx;
   P.ly = g->ly;
   P.lz = g->lz;
@@ -228,9 +460,74 @@ int make_params(cbct_ctx ctx, const cbct_geom *g, float spp, FPParams &P) {
   return CBCT_OK;
 }
 
+template <typename T, int RPL>
+void launch_col(const FPParams &P, hipStream_t s, const ViewD *views, const T *vt, float *proj,
+                int acc, int nviews) {
+  const unsigned ngroups = (unsigned)((P.nv + 64 * RPL - 1) / (64 * RPL));
+  const unsigned nblocks = (unsigned)P.nu * ngroups * (unsigned)nviews;
+  hipLaunchKernelGGL((k_fp_col<T, RPL>), dim3(nblocks), dim3(64), P.pz * sizeof(float), s, P, views,
+                     vt, proj, acc, ngroups, nblocks);
+}
+
+template <typename T>
+void dispatch_col(const FPParams &P, hipStream_t s, const ViewD *views, const T *vt, float *proj,
+                  int acc, int nviews) {
+  const int need = (P.nv + 63) / 64;
+  if (need <= 1) launch_col<T, 1>(P, s, views, vt, proj, acc, nviews);
+  else if (need <= 2) launch_col<T, 2>(P, s, views, vt, proj, acc, nviews);
+  else if (need <= 4) launch_col<T, 4>(P, s, views, vt, proj, acc, nviews);
+  else if (need <= 8) launch_col<T, 8>(P, s, views, vt, proj, acc, nviews);
+  else launch_col<T, 12>(P, s, views, vt, proj, acc, nviews);
+}
+
+template <typename T>
+int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
+           const T *vol, float *proj, int accumulate, hipStream_t s) {
+  const size_t esz = sizeof(T);
+  const ViewD *views = ctx->d_views_d + view0;
+  if (ctx->fp_engine == 0) {
+    const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
+    int rc = cbct_reserve(ctx, ctx->vol_pad, npad * esz);
+    if (rc) return rc;
+    T *vp = static_cast<T *>(ctx->vol_pad.ptr);
+    hipLaunchKernelGGL(k_pad_volume<T>, stride_grid(npad), dim3(256), 0, s, vol, vp, g->nx, g->ny,
+                       g->nz, P.px, (unsigned)padded_dim(g->ny), (unsigned)padded_dim(g->nz));
+    CBCT_LAUNCH_CHECK(ctx, "k_pad_volume");
+    dim3 block(16, 16);
+    dim3 grid(ceil_div(g->nu, 16), ceil_div(g->nv, 16), nviews);
+    cbct_prof_mark(ctx, PROF_FP, 0, s);
+    hipLaunchKernelGGL(k_fp_ray<T>, grid, block, 0, s, P, views, vp, proj, accumulate);
+    cbct_prof_mark(ctx, PROF_FP, 1, s);
+    CBCT_LAUNCH_CHECK(ctx, "k_fp_ray");
+    return CBCT_OK;
+  }
+  const size_t py = padded_dim(g->ny);
+  const size_t npad = (size_t)P.pzx * py;
+  int rc = cbct_reserve(ctx, ctx->vol_pad, npad * esz);
+  if (rc) return rc;
+  T *vt = static_cast<T *>(ctx->vol_pad.ptr);
+  dim3 tgrid(ceil_div(P.pz, 32), ceil_div(P.px, 32), (unsigned)py);
+  hipLaunchKernelGGL(k_pad_transpose<T>, tgrid, dim3(32, 8), 0, s, vol, vt, g->nx, g->ny, g->nz,
+                     P.px, P.pz);
+  CBCT_LAUNCH_CHECK(ctx, "k_pad_transpose");
+  cbct_prof_mark(ctx, PROF_FP, 0, s);
+  dispatch_col<T>(P, s, views, vt, proj, accumulate, nviews);
+  cbct_prof_mark(ctx, PROF_FP, 1, s);
+  CBCT_LAUNCH_CHECK(ctx, "k_fp_col");
+  return CBCT_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int cbct_fp_engine(cbct_ctx ctx, int engine) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (engine != 0 && engine != 1)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_fp_engine: engine %d not available", engine);
+  ctx->fp_engine = engine;
+  return CBCT_OK;
+}
 
 int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol, int dtype,
             float *proj, float spp, int accumulate, void *stream) {
@@ -247,40 +544,18 @@ int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
   if (nviews == 0) return CBCT_OK;
   if (nviews > 65535)
     return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: at most 65535 views per call", who);
+  if ((size_t)g->nu * (size_t)((g->nv + 63) / 64) * (size_t)nviews >= (1ull << 31))
+    return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: too many rays per call", who);
   FPParams P;
   rc = make_params(ctx, g, spp, P);
   if (rc) return rc;
   rc = cbct_set_device(ctx);
   if (rc) return rc;
-  const size_t esz = dtype == CBCT_F32 ? 4 : 2;
-  const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
-  rc = cbct_reserve(ctx, ctx->vol_pad, npad * esz);
-  if (rc) return rc;
   hipStream_t s = as_stream(stream);
-  const unsigned px = P.px, py = (unsigned)padded_dim(g->ny), pz = (unsigned)padded_dim(g->nz);
-  dim3 block(16, 16);
-  dim3 grid(ceil_div(g->nu, 16), ceil_div(g->nv, 16), nviews);
-  if (dtype == CBCT_F32) {
-    float *vp = static_cast<float *>(ctx->vol_pad.ptr);
-    hipLaunchKernelGGL(k_pad_volume<float>, stride_grid(npad), dim3(256), 0, s,
-                       static_cast<const float *>(vol), vp, g->nx, g->ny, g->nz, px, py, pz);
-    CBCT_LAUNCH_CHECK(ctx, "k_pad_volume");
-    cbct_prof_mark(ctx, PROF_FP, 0, s);
-    hipLaunchKernelGGL(k_fp_ray<float>, grid, block, 0, s, P, ctx->d_views_d + view0, vp, proj,
-                       accumulate);
-    cbct_prof_mark(ctx, PROF_FP, 1, s);
-  } else {
-    unsigned short *vp = static_cast<unsigned short *>(ctx->vol_pad.ptr);
-    hipLaunchKernelGGL(k_pad_volume<unsigned short>, stride_grid(npad), dim3(256), 0, s,
-                       static_cast<const unsigned short *>(vol), vp, g->nx, g->ny, g->nz, px, py, pz);
-    CBCT_LAUNCH_CHECK(ctx, "k_pad_volume");
-    cbct_prof_mark(ctx, PROF_FP, 0, s);
-    hipLaunchKernelGGL(k_fp_ray<unsigned short>, grid, block, 0, s, P, ctx->d_views_d + view0, vp,
-                       proj, accumulate);
-    cbct_prof_mark(ctx, PROF_FP, 1, s);
-  }
-  CBCT_LAUNCH_CHECK(ctx, "k_fp_ray");
-  return CBCT_OK;
+  if (dtype == CBCT_F32)
+    return run_fp<float>(ctx, g, P, view0, nviews, static_cast<const float *>(vol), proj, accumulate, s);
+  return run_fp<unsigned short>(ctx, g, P, view0, nviews, static_cast<const unsigned short *>(vol),
+                                proj, accumulate, s);
 }
 
 int cbct_fp_transpose(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *proj,

@@ -101,6 +101,7 @@ def load():
         "cbct_fp": (I, [P, G, I, I, P, I, P, F, I, P]),
         "cbct_bp": (I, [P, G, I, I, P, I, P, I, I, P]),
         "cbct_fp_transpose": (I, [P, G, I, I, P, P, F, I, P]),
+        "cbct_fp_engine": (I, [P, I]),
         "cbct_spacecarver": (I, [P, G, I, I, P, P, F, P]),
         "cbct_apply_mask": (I, [P, S, P, P, P]),
         "cbct_cosine_weights": (I, [G, P]),
@@ -232,6 +233,10 @@ class Context:
                                      _dtype_code(vol), _ptr(proj), samples_per_pixel,
                                      int(accumulate), _stream(stream)))
         return proj
+
+    def fp_engine(self, engine):
+        """1 = detector-column sharing (default), 0 = one thread per ray."""
+        self._check(self.lib.cbct_fp_engine(self.h, int(engine)))
 
     def bp(self, g: Geom, proj, vol, view0=0, nviews=None, fbp=False, accumulate=False,
            stream=None):

@@ -96,6 +96,12 @@ int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
 int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *proj,
             int proj_dtype, float *vol, int fbp, int accumulate, void *stream);
 
+/* Forward-projector engine: 1 = detector-column sharing (default; one wave per
+ * detector column interpolates the shared z-line once per step), 0 = one
+ * thread per ray.  Both give bitwise-identical projections.  No reference
+ * counterpart (the reference has one texture-based kernel). */
+int cbct_fp_engine(cbct_ctx ctx, int engine);
+
 /* Exact transpose of cbct_fp (ray-driven scatter, fp32 atomics):
  * vol = (accumulate ? vol : 0) + A^T proj.  No reference counterpart: the
  * reference (mult_M, mult_MH) pair is not adjoint; this exists for the

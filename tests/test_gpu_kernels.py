@@ -76,6 +76,35 @@ def test_fp_parity(ctx, name):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+@pytest.mark.parametrize("name", list(CASES) + ["tall", "wide"])
+def test_fp_engines_bitwise_identical(ctx, name, dtype):
+    """The detector-column engine and the per-ray engine give identical bits.
+    'tall' has most rays leaving through the top/bottom faces (per-ray
+    fallback); 'wide' has detector columns that miss the box entirely."""
+    if name == "tall":
+        g, angles, offsets = geom(20, 20, 5, 30, 150, fov=(40.0, 60.0)), angles_full(7), None
+    elif name == "wide":
+        g, angles, offsets = geom(12, 12, 12, 40, 70, fov=(400.0, 30.0)), angles_full(5), None
+    else:
+        g, angles, offsets = make_case(name)
+    print_case("engines", name, dtype, g)
+    vol = phantom_volume(g) + 0.01
+    tdt = torch.float32 if dtype == "f32" else torch.bfloat16
+    ctx.set_views(angles, offsets)
+    outs = []
+    for e in (0, 1):
+        ctx.fp_engine(e)
+        out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
+        ctx.fp(g, dev(vol, tdt), out)
+        torch.cuda.synchronize()
+        outs.append(out)
+    ctx.fp_engine(1)
+    assert torch.equal(outs[0], outs[1])
+    if dtype == "f32":
+        assert rel_l2(host(outs[1]), O.fp(to_oracle(g), angles, offsets, vol)) <= TOL
+
+
 @pytest.mark.parametrize("fbp", [False, True])
 @pytest.mark.parametrize("name", list(CASES))
 def test_bp_parity(ctx, name, fbp):
@@ -279,7 +308,7 @@ def test_elementwise_and_reductions(ctx):
     ctx.sart_update(XS, dev(img), dev(wx), 0.7, True)
     assert rel_l2(host(XS), np.maximum(xs + 0.7 * img * wx, 0.0)) <= 1e-6
     z, xo = rng.random(n), rng.random(n)
-    xx = rng.standard_normal(n)
+    xx = rng.standard_normal(n).astype(np.float32).astype(np.float64)
     XX, Z = dev(xx), dev(z)
     ctx.mom_momentum(XX, dev(xo), Z, 0.4, True)
     xc = np.maximum(xx, 0)
