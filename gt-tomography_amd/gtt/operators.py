@@ -1,0 +1,163 @@
+"""ctypes binding of libgtt.so (include/gtt_c.h): the C++ operator/solver
+layer -- FDK_reconstruct_3d, osSARTSolver and osMOMSolverD over
+CBSubsetOperator, and the subset operator itself."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import capi
+
+_lib = None
+
+
+def lib_path():
+    return os.path.join(capi.LIBDIR, "libgtt.so")
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    capi.load()
+    if not os.path.exists(lib_path()):
+        raise ImportError("libgtt.so not built (run __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(lib_path())
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    F = ctypes.c_float
+    G = ctypes.POINTER(capi.Geom)
+    D = ctypes.POINTER(ctypes.c_double)
+    L.gtt_last_error.restype = ctypes.c_char_p
+    L.gtt_last_error.argtypes = []
+    L.gtt_fdk_reconstruct_3d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, D]
+    L.gtt_os_sart.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, P, D]
+    L.gtt_os_mom.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, F, F, I, I, P]
+    L.gtt_subset_op_create.argtypes = [I, ctypes.POINTER(P)]
+    L.gtt_subset_op_destroy.argtypes = [P]
+    L.gtt_subset_op_setup.argtypes = [P, G, P, P, I, P, P, I, P, P, P]
+    L.gtt_subset_op_views.argtypes = [P, I]
+    L.gtt_subset_op_mult_M.argtypes = [P, P, P, I, I]
+    L.gtt_subset_op_mult_MH.argtypes = [P, P, P, I, I]
+    for n in ("gtt_fdk_reconstruct_3d", "gtt_os_sart", "gtt_os_mom", "gtt_subset_op_create",
+              "gtt_subset_op_destroy", "gtt_subset_op_setup", "gtt_subset_op_views",
+              "gtt_subset_op_mult_M", "gtt_subset_op_mult_MH"):
+        getattr(L, n).restype = I
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(load().gtt_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _views(angles, offsets):
+    a = np.ascontiguousarray(angles, np.float32).ravel()
+    o = None if offsets is None else np.ascontiguousarray(offsets, np.float32).reshape(-1)
+    return a, o
+
+
+def _bins(bins):
+    if bins is None:
+        return None, None, 0
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(b, np.uint32) for b in bins]) if bins else
+                               np.zeros(0, np.uint32), np.uint32)
+    sizes = np.ascontiguousarray([len(b) for b in bins], np.int32)
+    return idx, sizes, len(bins)
+
+
+def _vol_shape(g, nbins):
+    return ((nbins,) if nbins > 1 else ()) + (g.nz, g.ny, g.nx)
+
+
+def fdk_reconstruct_3d(g: capi.Geom, angles, offsets, projections, filt=None, bins=None,
+                       samples_per_pixel=1.5):
+    """FDK_reconstruct_3d: returns (volume, least-squares scale)."""
+    L = load()
+    a, o = _views(angles, offsets)
+    idx, sizes, nb = _bins(bins)
+    p = np.ascontiguousarray(projections, np.float32)
+    F = None if filt is None else np.ascontiguousarray(filt, np.float32)
+    out = np.zeros(_vol_shape(g, nb), np.float32)
+    scale = ctypes.c_double()
+    _check(L.gtt_fdk_reconstruct_3d(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb,
+                                    _p(p), _p(F), samples_per_pixel, _p(out), ctypes.byref(scale)))
+    return out, scale.value
+
+
+def os_sart(g: capi.Geom, angles, offsets, projections, nsubsets, iterations, beta=1.0, gamma=0.0,
+            nonneg=False, bins=None, objective=False):
+    """osSARTSolver over CBSubsetOperator; projections in acquisition order."""
+    L = load()
+    a, o = _views(angles, offsets)
+    idx, sizes, nb = _bins(bins)
+    p = np.ascontiguousarray(projections, np.float32)
+    out = np.zeros(_vol_shape(g, nb), np.float32)
+    obj = np.zeros(iterations, np.float64) if objective else None
+    _check(L.gtt_os_sart(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb, _p(p),
+                         nsubsets, iterations, beta, gamma, int(nonneg), _p(out),
+                         None if obj is None else obj.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return (out, obj) if objective else out
+
+
+def os_mom(g: capi.Geom, angles, offsets, projections, nsubsets, iterations, tv_weight=0.0,
+           tau=1e-3, reg_steps=1, beta=1.0, gamma=0.0, nonneg=False, fuse_tv=True, bins=None):
+    """osMOMSolverD (+ isotropic TV) over CBSubsetOperator; returns z."""
+    L = load()
+    a, o = _views(angles, offsets)
+    idx, sizes, nb = _bins(bins)
+    p = np.ascontiguousarray(projections, np.float32)
+    out = np.zeros(_vol_shape(g, nb), np.float32)
+    _check(L.gtt_os_mom(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb, _p(p),
+                        nsubsets, iterations, tv_weight, tau, reg_steps, beta, gamma, int(nonneg),
+                        int(fuse_tv), _p(out)))
+    return out
+
+
+class CBSubsetOperator:
+    """CBSubsetOperator<cuNDArray> on torch device tensors (default stream)."""
+
+    def __init__(self, nsubsets):
+        self.L = load()
+        h = ctypes.c_void_p()
+        _check(self.L.gtt_subset_op_create(int(nsubsets), ctypes.byref(h)))
+        self.h = h
+        self.nsubsets = nsubsets
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.gtt_subset_op_destroy(self.h)
+        except Exception:
+            pass
+
+    def setup(self, g: capi.Geom, angles, offsets, projections=None, bins=None):
+        """Returns (permuted projections or None, permutation)."""
+        self.g = g
+        a, o = _views(angles, offsets)
+        idx, sizes, nb = _bins(bins)
+        self.nbins = max(nb, 1)
+        p = None if projections is None else np.ascontiguousarray(projections, np.float32)
+        perm_p = None if p is None else np.empty_like(p)
+        perm = np.zeros(a.size, np.uint32)
+        _check(self.L.gtt_subset_op_setup(self.h, ctypes.byref(g), _p(a), _p(o), a.size, _p(idx),
+                                          _p(sizes), nb, _p(p), _p(perm_p), _p(perm)))
+        return perm_p, perm
+
+    def views(self, subset=-1):
+        return self.L.gtt_subset_op_views(self.h, int(subset))
+
+    def mult_M(self, vol, proj, subset=-1, accumulate=False):
+        _check(self.L.gtt_subset_op_mult_M(self.h, ctypes.c_void_p(vol.data_ptr()),
+                                           ctypes.c_void_p(proj.data_ptr()), int(subset), int(accumulate)))
+
+    def mult_MH(self, proj, vol, subset=-1, accumulate=False):
+        _check(self.L.gtt_subset_op_mult_MH(self.h, ctypes.c_void_p(proj.data_ptr()),
+                                            ctypes.c_void_p(vol.data_ptr()), int(subset), int(accumulate)))

@@ -1,0 +1,239 @@
+// C entry points of libgtt (include/gtt_c.h).
+#include "gtt_c.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gtt/fdk.h"
+#include "gtt/operators.h"
+#include "gtt/solvers.h"
+
+namespace gtt {
+int current_device() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess) return 0;
+  return d;
+}
+}  // namespace gtt
+
+using namespace gtt;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Scan {
+  std::shared_ptr<CBCT_acquisition> acq;
+  std::shared_ptr<CBCT_binning> binning;
+  floatd3 dims_mm;
+  std::vector<size_t> vol_dims;
+};
+
+Scan make_scan(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
+               const unsigned *bin_index, const int *bin_sizes, int nbins, const float *proj) {
+  if (!g || !angles || nviews <= 0) throw std::runtime_error("illegal geometry or angle array provided");
+  Scan s;
+  auto geom = std::make_shared<CBCT_geometry>();
+  geom->set_SAD(g->sad);
+  geom->set_SDD(g->sdd);
+  geom->set_FOV(floatd2{{g->fov_u, g->fov_v}});
+  geom->set_angles(std::vector<float>(angles, angles + nviews));
+  std::vector<floatd2> o(nviews, floatd2{{0, 0}});
+  if (offsets)
+    for (int i = 0; i < nviews; ++i) o[i] = floatd2{{offsets[2 * i], offsets[2 * i + 1]}};
+  geom->set_offsets(o);
+  std::shared_ptr<hoCuNDArray<float>> p;
+  if (proj) {
+    p = std::make_shared<hoCuNDArray<float>>(std::vector<size_t>{(size_t)g->nu, (size_t)g->nv, (size_t)nviews});
+    std::memcpy(p->get_data_ptr(), proj, p->get_number_of_bytes());
+  }
+  s.acq = std::make_shared<CBCT_acquisition>(p, geom);
+  s.binning = std::make_shared<CBCT_binning>();
+  if (nbins <= 0) {
+    s.binning->set_as_default_3d_bin((unsigned)nviews);
+    nbins = 1;
+  } else {
+    std::vector<std::vector<unsigned int>> bins(nbins);
+    size_t k = 0;
+    for (int b = 0; b < nbins; ++b)
+      for (int i = 0; i < bin_sizes[b]; ++i) bins[b].push_back(bin_index[k++]);
+    s.binning = std::make_shared<CBCT_binning>(bins);
+  }
+  s.dims_mm = floatd3{{g->lx, g->ly, g->lz}};
+  s.vol_dims = {(size_t)g->nx, (size_t)g->ny, (size_t)g->nz};
+  if (nbins > 1) s.vol_dims.push_back((size_t)nbins);
+  return s;
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+}  // namespace
+
+struct gtt_subset_op_s {
+  std::shared_ptr<CBSubsetOperator> op;
+};
+
+extern "C" {
+
+const char *gtt_last_error(void) { return g_err.c_str(); }
+
+int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
+                           const unsigned *bin_index, const int *bin_sizes, int nbins,
+                           const float *projections, const float *filter, float spp,
+                           float *volume_out, double *scale_out) {
+  return guarded([&] {
+    if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    std::shared_ptr<std::vector<float>> F;
+    if (filter) F = std::make_shared<std::vector<float>>(filter, filter + g->nu + 1);
+    FDKResult r = FDK_reconstruct_3d(s.acq, s.binning, s.vol_dims, s.dims_mm, F, spp);
+    std::memcpy(volume_out, r.image->get_data_ptr(), r.image->get_number_of_bytes());
+    if (scale_out) *scale_out = r.scale;
+  });
+}
+
+int gtt_os_sart(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
+                const unsigned *bin_index, const int *bin_sizes, int nbins, const float *projections,
+                int nsubsets, int iterations, float beta, float gamma, int nonneg, float *volume_out,
+                double *objective_out) {
+  return guarded([&] {
+    if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
+    if (nsubsets <= 0) throw std::runtime_error("number of subsets must be positive");
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    auto E = std::make_shared<CBSubsetOperator>(nsubsets);
+    E->setup(s.acq, s.binning, s.dims_mm);
+    E->set_domain_dimensions(&s.vol_dims);
+    cuNDArray<float> b(*s.acq->get_projections());
+    osSARTSolver<cuNDArray<float>> solver;
+    solver.set_encoding_operator(E);
+    solver.set_max_iterations(iterations);
+    solver.set_beta(beta);
+    solver.set_gamma(gamma);
+    solver.set_non_negativity_constraint(nonneg != 0);
+    if (objective_out)
+      solver.set_objective_callback([&](int it, double v) { objective_out[it] = v; });
+    auto x = solver.solve(&b);
+    copy_d2h(volume_out, x->get_data_ptr(), x->get_number_of_bytes());
+  });
+}
+
+int gtt_os_mom(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
+               const unsigned *bin_index, const int *bin_sizes, int nbins, const float *projections,
+               int nsubsets, int iterations, float tv_weight, float tau, int reg_steps, float beta,
+               float gamma, int nonneg, int fuse_tv, float *volume_out) {
+  return guarded([&] {
+    if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
+    if (nsubsets <= 0) throw std::runtime_error("number of subsets must be positive");
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    auto E = std::make_shared<CBSubsetOperator>(nsubsets);
+    E->setup(s.acq, s.binning, s.dims_mm);
+    E->set_domain_dimensions(&s.vol_dims);
+    cuNDArray<float> b(*s.acq->get_projections());
+    osMOMSolverD<cuNDArray<float>> solver;
+    solver.set_dump(false);
+    solver.set_encoding_operator(E);
+    solver.set_max_iterations(iterations);
+    solver.set_tau(tau);
+    solver.set_non_negativity_constraint(nonneg != 0);
+    solver.set_reg_steps((unsigned)reg_steps);
+    solver.set_beta(beta);
+    solver.set_gamma(gamma);
+    solver.set_fuse_tv(fuse_tv != 0);
+    if (tv_weight > 0) {
+      auto TV = std::make_shared<cuTVPrimalDualOperator>(0.0f);
+      TV->set_weight(tv_weight);
+      solver.add_regularization_operator(TV);
+    }
+    auto z = solver.solve(&b);
+    copy_d2h(volume_out, z->get_data_ptr(), z->get_number_of_bytes());
+  });
+}
+
+int gtt_subset_op_create(int nsubsets, gtt_subset_op *out) {
+  return guarded([&] {
+    if (!out || nsubsets <= 0) throw std::runtime_error("gtt_subset_op_create: bad arguments");
+    auto *h = new gtt_subset_op_s();
+    h->op = std::make_shared<CBSubsetOperator>(nsubsets);
+    *out = h;
+  });
+}
+
+int gtt_subset_op_destroy(gtt_subset_op op) {
+  delete op;
+  return 0;
+}
+
+int gtt_subset_op_setup(gtt_subset_op op, const cbct_geom *g, const float *angles,
+                        const float *offsets, int nviews, const unsigned *bin_index,
+                        const int *bin_sizes, int nbins, const float *projections,
+                        float *permuted_out, unsigned *permutation_out) {
+  return guarded([&] {
+    if (!op) throw std::runtime_error("null operator");
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    if (!projections)  // projection dims still come from the geometry
+      s.acq->set_projections(std::make_shared<hoCuNDArray<float>>(
+          std::vector<size_t>{(size_t)g->nu, (size_t)g->nv, (size_t)nviews}));
+    op->op->setup(s.acq, s.binning, s.dims_mm);
+    op->op->set_domain_dimensions(&s.vol_dims);
+    auto p = s.acq->get_projections();
+    if (permuted_out && projections) std::memcpy(permuted_out, p->get_data_ptr(), p->get_number_of_bytes());
+    if (permutation_out) {
+      const auto &perm = op->op->get_permutation();
+      for (size_t i = 0; i < perm.size(); ++i) permutation_out[i] = perm[i];
+    }
+  });
+}
+
+int gtt_subset_op_views(gtt_subset_op op, int subset) {
+  if (!op) return -1;
+  try {
+    if (subset < 0) return (int)op->op->get_codomain_dimensions()->back();
+    return (int)op->op->get_codomain_dimensions(subset)->back();
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int gtt_subset_op_mult_M(gtt_subset_op op, float *vol, float *proj, int subset, int accumulate) {
+  return guarded([&] {
+    auto dom = *op->op->get_domain_dimensions();
+    cuNDArray<float> x(dom, vol);
+    if (subset < 0) {
+      cuNDArray<float> y(*op->op->get_codomain_dimensions(), proj);
+      op->op->mult_M(&x, &y, accumulate != 0);
+    } else {
+      cuNDArray<float> y(*op->op->get_codomain_dimensions(subset), proj);
+      op->op->mult_M(&x, &y, subset, accumulate != 0);
+    }
+  });
+}
+
+int gtt_subset_op_mult_MH(gtt_subset_op op, float *proj, float *vol, int subset, int accumulate) {
+  return guarded([&] {
+    auto dom = *op->op->get_domain_dimensions();
+    cuNDArray<float> x(dom, vol);
+    if (subset < 0) {
+      cuNDArray<float> y(*op->op->get_codomain_dimensions(), proj);
+      op->op->mult_MH(&y, &x, accumulate != 0);
+    } else {
+      cuNDArray<float> y(*op->op->get_codomain_dimensions(subset), proj);
+      op->op->mult_MH(&y, &x, subset, accumulate != 0);
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+// Device memory helpers and the cuNDArray math used by the operators and
+// solvers, on top of libcbct.
+#include "gtt/ndarray.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+
+namespace gtt {
+
+namespace {
+std::mutex g_mu;
+std::map<int, cbct_ctx> g_ctx;
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace
+
+cbct_ctx device_context() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_ctx.find(dev);
+  if (it != g_ctx.end()) return it->second;
+  cbct_ctx c = nullptr;
+  int rc = cbct_ctx_create(dev, &c);
+  if (rc) throw std::runtime_error(cbct_last_error(nullptr));
+  g_ctx[dev] = c;
+  return c;
+}
+
+void check(int rc, cbct_ctx ctx) {
+  if (rc != CBCT_OK) throw std::runtime_error(cbct_last_error(ctx));
+}
+
+void device_malloc(void **p, size_t bytes) { hip_check(hipMalloc(p, bytes), "hipMalloc"); }
+void device_free(void *p) { (void)hipFree(p); }
+void copy_d2d(void *dst, const void *src, size_t bytes) {
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, nullptr), "hipMemcpyAsync");
+}
+void copy_h2d(void *dst, const void *src, size_t bytes) {
+  hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+void copy_d2h(void *dst, const void *src, size_t bytes) {
+  hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+void device_synchronize() { hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+#define CTX device_context()
+
+void clear(cuNDArray<float> *x) { fill(x, 0.0f); }
+void fill(cuNDArray<float> *x, float v) {
+  cbct_ctx c = CTX;
+  check(cbct_fill(c, x->get_number_of_elements(), x->get_data_ptr(), v, nullptr), c);
+}
+void clamp_min(cuNDArray<float> *x, float v) {
+  cbct_ctx c = CTX;
+  check(cbct_clamp_min(c, x->get_number_of_elements(), x->get_data_ptr(), v, nullptr), c);
+}
+void reciprocal_inplace(cuNDArray<float> *x) {
+  cbct_ctx c = CTX;
+  check(cbct_reciprocal(c, x->get_number_of_elements(), x->get_data_ptr(), nullptr), c);
+}
+static void same_size(const cuNDArray<float> *a, const cuNDArray<float> *b, const char *who) {
+  if (a->get_number_of_elements() != b->get_number_of_elements())
+    throw std::runtime_error(std::string(who) + ": array sizes differ");
+}
+void axpy(float a, const cuNDArray<float> *x, cuNDArray<float> *y) {
+  same_size(x, y, "axpy");
+  cbct_ctx c = CTX;
+  check(cbct_axpy(c, y->get_number_of_elements(), a, x->get_data_ptr(), y->get_data_ptr(), nullptr), c);
+}
+double dot(const cuNDArray<float> *a, const cuNDArray<float> *b) {
+  same_size(a, b, "dot");
+  cbct_ctx c = CTX;
+  double r = 0;
+  check(cbct_reduce_f64(c, CBCT_DOT, a->get_number_of_elements(), a->get_data_ptr(), b->get_data_ptr(), &r, nullptr), c);
+  return r;
+}
+double nrm2(const cuNDArray<float> *a) {
+  cbct_ctx c = CTX;
+  double r = 0;
+  check(cbct_reduce_f64(c, CBCT_NRM2, a->get_number_of_elements(), a->get_data_ptr(), nullptr, &r, nullptr), c);
+  return r;
+}
+double asum(const cuNDArray<float> *a) {
+  cbct_ctx c = CTX;
+  double r = 0;
+  check(cbct_reduce_f64(c, CBCT_ASUM, a->get_number_of_elements(), a->get_data_ptr(), nullptr, &r, nullptr), c);
+  return r;
+}
+
+void sart_residual(cuNDArray<float> *r, const cuNDArray<float> *b, const cuNDArray<float> *w) {
+  same_size(r, b, "sart_residual");
+  same_size(r, w, "sart_residual");
+  cbct_ctx c = CTX;
+  check(cbct_sart_residual(c, r->get_number_of_elements(), r->get_data_ptr(), b->get_data_ptr(),
+                           w->get_data_ptr(), nullptr), c);
+}
+void sart_update(cuNDArray<float> *x, const cuNDArray<float> *img, const cuNDArray<float> *wx,
+                 float step, bool nonneg) {
+  same_size(x, img, "sart_update");
+  same_size(x, wx, "sart_update");
+  cbct_ctx c = CTX;
+  check(cbct_sart_update(c, x->get_number_of_elements(), x->get_data_ptr(), img->get_data_ptr(),
+                         wx->get_data_ptr(), step, nonneg ? 1 : 0, nullptr), c);
+}
+void mom_gradient_step(cuNDArray<float> *z, const cuNDArray<float> *img,
+                       const cuNDArray<float> *precon, float step) {
+  same_size(z, img, "mom_gradient_step");
+  same_size(z, precon, "mom_gradient_step");
+  cbct_ctx c = CTX;
+  check(cbct_mom_gradient_step(c, z->get_number_of_elements(), z->get_data_ptr(), img->get_data_ptr(),
+                               precon->get_data_ptr(), step, nullptr), c);
+}
+void mom_momentum(cuNDArray<float> *x, const cuNDArray<float> *xold, cuNDArray<float> *z, float cc,
+                  bool nonneg) {
+  same_size(x, xold, "mom_momentum");
+  same_size(x, z, "mom_momentum");
+  cbct_ctx c = CTX;
+  check(cbct_mom_momentum(c, x->get_number_of_elements(), x->get_data_ptr(), xold->get_data_ptr(),
+                          z->get_data_ptr(), cc, nonneg ? 1 : 0, nullptr), c);
+}
+
+template <>
+cuNDArray<float> &cuNDArray<float>::operator*=(float a) {
+  cbct_ctx c = CTX;
+  check(cbct_scale(c, get_number_of_elements(), data_, a, nullptr), c);
+  return *this;
+}
+template <>
+cuNDArray<float> &cuNDArray<float>::operator+=(float a) {
+  cbct_ctx c = CTX;
+  check(cbct_add_scalar(c, get_number_of_elements(), data_, a, nullptr), c);
+  return *this;
+}
+template <>
+cuNDArray<float> &cuNDArray<float>::operator*=(const cuNDArray<float> &a) {
+  same_size(this, &a, "operator*=");
+  cbct_ctx c = CTX;
+  check(cbct_mul(c, get_number_of_elements(), a.data_, data_, nullptr), c);
+  return *this;
+}
+template <>
+cuNDArray<float> &cuNDArray<float>::operator/=(const cuNDArray<float> &a) {
+  same_size(this, &a, "operator/=");
+  cbct_ctx c = CTX;
+  check(cbct_div(c, get_number_of_elements(), a.data_, data_, nullptr), c);
+  return *this;
+}
+template <>
+cuNDArray<float> &cuNDArray<float>::operator+=(const cuNDArray<float> &a) {
+  same_size(this, &a, "operator+=");
+  cbct_ctx c = CTX;
+  check(cbct_add(c, get_number_of_elements(), a.data_, data_, nullptr), c);
+  return *this;
+}
+template <>
+cuNDArray<float> &cuNDArray<float>::operator-=(const cuNDArray<float> &a) {
+  same_size(this, &a, "operator-=");
+  cbct_ctx c = CTX;
+  check(cbct_sub(c, get_number_of_elements(), a.data_, data_, nullptr), c);
+  return *this;
+}
+
+void clear(hoCuNDArray<float> *x) {
+  float *p = x->get_data_ptr();
+  for (size_t i = 0, n = x->get_number_of_elements(); i < n; ++i) p[i] = 0.0f;
+}
+
+double dot(const hoCuNDArray<float> *a, const hoCuNDArray<float> *b) {
+  if (a->get_number_of_elements() != b->get_number_of_elements())
+    throw std::runtime_error("dot: array sizes differ");
+  double acc = 0.0;
+  const float *pa = a->get_data_ptr(), *pb = b->get_data_ptr();
+  for (size_t i = 0, n = a->get_number_of_elements(); i < n; ++i) acc += (double)pa[i] * pb[i];
+  return acc;
+}
+
+}  // namespace gtt

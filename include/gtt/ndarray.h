@@ -1,0 +1,185 @@
+// Device and host N-d arrays with the Gadgetron array conventions the
+// reference's operators and solvers use (first index fastest, dimension
+// vectors, views onto foreign memory, deep copy on copy-construction) and the
+// element-wise / BLAS free functions of cuNDArray_math / cuNDArray_blas.
+// Device arithmetic runs through libcbct (include/cbct.h) on the default HIP
+// stream of the current thread, like the reference's legacy default stream.
+#pragma once
+
+#include <cstddef>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cbct.h"
+
+namespace gtt {
+
+// Per-device context used by the array math (one per process and device).
+cbct_ctx device_context();
+// HIP device current on the calling thread.
+int current_device();
+// Throw std::runtime_error with the library's message when rc != 0.
+void check(int rc, cbct_ctx ctx);
+
+void device_malloc(void **p, size_t bytes);
+void device_free(void *p);
+void copy_d2d(void *dst, const void *src, size_t bytes);
+void copy_h2d(void *dst, const void *src, size_t bytes);
+void copy_d2h(void *dst, const void *src, size_t bytes);
+void device_synchronize();
+
+inline size_t prod(const std::vector<size_t> &d) {
+  return std::accumulate(d.begin(), d.end(), size_t(1), std::multiplies<size_t>());
+}
+
+template <class T>
+class hoCuNDArray;
+
+// Device array.  Owning unless constructed as a view on foreign memory.
+template <class T>
+class cuNDArray {
+ public:
+  typedef T element_type;
+
+  cuNDArray() = default;
+  explicit cuNDArray(const std::vector<size_t> &dims) { create(dims); }
+  explicit cuNDArray(const std::vector<size_t> *dims) { create(*dims); }
+  explicit cuNDArray(std::shared_ptr<std::vector<size_t>> dims) { create(*dims); }
+  cuNDArray(const std::vector<size_t> &dims, T *data) : dims_(dims), data_(data), owns_(false) {}
+  cuNDArray(const cuNDArray &o) { *this = o; }
+  cuNDArray(cuNDArray &&o) noexcept { swap(o); }
+  explicit cuNDArray(const hoCuNDArray<T> &h);
+  ~cuNDArray() { release(); }
+
+  cuNDArray &operator=(const cuNDArray &o) {
+    if (this == &o) return *this;
+    if (!data_ || prod(dims_) != o.get_number_of_elements()) create(o.dims_);
+    dims_ = o.dims_;
+    if (o.get_number_of_elements()) copy_d2d(data_, o.data_, o.get_number_of_bytes());
+    return *this;
+  }
+  cuNDArray &operator=(cuNDArray &&o) noexcept {
+    swap(o);
+    return *this;
+  }
+
+  void create(const std::vector<size_t> &dims) {
+    release();
+    dims_ = dims;
+    const size_t n = prod(dims_);
+    if (n) device_malloc(reinterpret_cast<void **>(&data_), n * sizeof(T));
+    owns_ = true;
+  }
+  void create(const std::vector<size_t> *dims) { create(*dims); }
+
+  std::shared_ptr<std::vector<size_t>> get_dimensions() const {
+    return std::make_shared<std::vector<size_t>>(dims_);
+  }
+  const std::vector<size_t> &dims() const { return dims_; }
+  size_t get_size(size_t i) const { return i < dims_.size() ? dims_[i] : 1; }
+  size_t get_number_of_dimensions() const { return dims_.size(); }
+  size_t get_number_of_elements() const { return dims_.empty() ? 0 : prod(dims_); }
+  size_t get_number_of_bytes() const { return get_number_of_elements() * sizeof(T); }
+  T *get_data_ptr() const { return data_; }
+  bool dimensions_equal(const cuNDArray *o) const { return o && o->dims_ == dims_; }
+  void to_host(hoCuNDArray<T> *out) const;
+
+  // Element-wise operators (float arrays)
+  cuNDArray &operator*=(T a);
+  cuNDArray &operator+=(T a);
+  cuNDArray &operator*=(const cuNDArray &a);
+  cuNDArray &operator/=(const cuNDArray &a);
+  cuNDArray &operator+=(const cuNDArray &a);
+  cuNDArray &operator-=(const cuNDArray &a);
+
+  void swap(cuNDArray &o) noexcept {
+    std::swap(dims_, o.dims_);
+    std::swap(data_, o.data_);
+    std::swap(owns_, o.owns_);
+  }
+
+ private:
+  void release() {
+    if (owns_ && data_) device_free(data_);
+    data_ = nullptr;
+    owns_ = false;
+  }
+  std::vector<size_t> dims_;
+  T *data_ = nullptr;
+  bool owns_ = false;
+};
+
+// Host array (the reference's page-locked hoCuNDArray).
+template <class T>
+class hoCuNDArray {
+ public:
+  typedef T element_type;
+  hoCuNDArray() = default;
+  explicit hoCuNDArray(const std::vector<size_t> &dims) : dims_(dims), own_(prod(dims)), data_(own_.data()) {}
+  explicit hoCuNDArray(const std::vector<size_t> *dims) : hoCuNDArray(*dims) {}
+  explicit hoCuNDArray(std::shared_ptr<std::vector<size_t>> dims) : hoCuNDArray(*dims) {}
+  hoCuNDArray(const std::vector<size_t> &dims, T *data) : dims_(dims), data_(data) {}
+  hoCuNDArray(const hoCuNDArray &o) : dims_(o.dims_), own_(o.data_, o.data_ + o.get_number_of_elements()), data_(own_.data()) {}
+  hoCuNDArray &operator=(const hoCuNDArray &o) {
+    if (this == &o) return *this;
+    dims_ = o.dims_;
+    own_.assign(o.data_, o.data_ + o.get_number_of_elements());
+    data_ = own_.data();
+    return *this;
+  }
+  std::shared_ptr<std::vector<size_t>> get_dimensions() const {
+    return std::make_shared<std::vector<size_t>>(dims_);
+  }
+  const std::vector<size_t> &dims() const { return dims_; }
+  size_t get_size(size_t i) const { return i < dims_.size() ? dims_[i] : 1; }
+  size_t get_number_of_dimensions() const { return dims_.size(); }
+  size_t get_number_of_elements() const { return dims_.empty() ? 0 : prod(dims_); }
+  size_t get_number_of_bytes() const { return get_number_of_elements() * sizeof(T); }
+  T *get_data_ptr() const { return data_; }
+
+ private:
+  std::vector<size_t> dims_;
+  std::vector<T> own_;
+  T *data_ = nullptr;
+};
+
+template <class T>
+cuNDArray<T>::cuNDArray(const hoCuNDArray<T> &h) {
+  create(h.dims());
+  if (get_number_of_elements()) copy_h2d(data_, h.get_data_ptr(), get_number_of_bytes());
+}
+
+template <class T>
+void cuNDArray<T>::to_host(hoCuNDArray<T> *out) const {
+  if (out->get_number_of_elements() != get_number_of_elements()) *out = hoCuNDArray<T>(dims_);
+  if (get_number_of_elements()) copy_d2h(out->get_data_ptr(), data_, get_number_of_bytes());
+}
+
+// cuNDArray_math / cuNDArray_blas equivalents (float).  dot/nrm2/asum
+// accumulate in FP64 on the device and return double.
+void clear(cuNDArray<float> *x);
+void fill(cuNDArray<float> *x, float v);
+void clamp_min(cuNDArray<float> *x, float v);
+void reciprocal_inplace(cuNDArray<float> *x);
+void axpy(float a, const cuNDArray<float> *x, cuNDArray<float> *y);
+double dot(const cuNDArray<float> *a, const cuNDArray<float> *b);
+double nrm2(const cuNDArray<float> *a);
+double asum(const cuNDArray<float> *a);
+
+// Fused solver passes (libcbct): see cbct_sart_residual etc.
+void sart_residual(cuNDArray<float> *r, const cuNDArray<float> *b, const cuNDArray<float> *w);
+void sart_update(cuNDArray<float> *x, const cuNDArray<float> *img, const cuNDArray<float> *wx,
+                 float step, bool nonneg);
+void mom_gradient_step(cuNDArray<float> *z, const cuNDArray<float> *img,
+                       const cuNDArray<float> *precon, float step);
+void mom_momentum(cuNDArray<float> *x, const cuNDArray<float> *xold, cuNDArray<float> *z, float c,
+                  bool nonneg);
+
+// Host helpers
+void clear(hoCuNDArray<float> *x);
+double dot(const hoCuNDArray<float> *a, const hoCuNDArray<float> *b);
+
+}  // namespace gtt
