@@ -46,6 +46,22 @@ __device__ __forceinline__ float ldp<unsigned short>(const unsigned short *p, un
   return bf16_to_f32(p[i]);
 }
 
+// Raw buffer load of one padded-projection texel (32-bit byte offsets within
+// one view; the descriptor covers exactly that view, so out-of-range offsets
+// read zero rather than another view).
+template <typename T>
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float bld<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  // the builtin returns the raw 32-bit word
+  return __uint_as_float((unsigned)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ float bld<unsigned short>(__amdgpu_buffer_rsrc_t r, unsigned voff,
+                                                     unsigned soff) {
+  return bf16_to_f32(__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0));
+}
+
 template <typename T, bool FBP, int KZ>
 __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__restrict__ views,
                                                      const T *__restrict__ ppad,
@@ -62,37 +78,41 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
   float acc[KZ];
 #pragma unroll
   for (int t = 0; t < KZ; ++t) acc[t] = 0.0f;
+  const unsigned esz = sizeof(T);
+  const unsigned row_bytes = P.pnu * esz;  // < 2^24: 24-bit multiplies suffice
+  const int view_bytes = (int)(P.pview * esz);
 
+  ViewF vw = views[0];
   for (int k = 0; k < P.nviews; ++k) {
-    const ViewF vw = views[k];
+    const ViewF nxt = views[k + 1 < P.nviews ? k + 1 : k];  // prefetch the next view
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(ppad + (size_t)k * P.pview), (short)0, view_bytes, 0x00020000);
     const float qx = fmaf(vw.c, px, vw.s * py);
     const float qy = fmaf(vw.c, py, -vw.s * px);
-    const float mag = P.sdd / (qy + P.sad);
-    float tu = fmaf(fmaf(qx, mag, -vw.off_u), P.su, P.cu);
-    tu = fminf(fmaxf(tu, 0.0f), P.clu);
-    const float flu = floorf(tu);
-    const float au = tu - flu;
-    const T *row_base = ppad + (size_t)k * P.pview + (unsigned)flu;
+    const float mag = P.sdd * __builtin_amdgcn_rcpf(qy + P.sad);
+    // coordinates are finite here; med3 = clamp, truncation = floor
+    const float tu = __builtin_amdgcn_fmed3f(fmaf(fmaf(qx, mag, -vw.off_u), P.su, P.cu), 0.0f, P.clu);
+    const float au = __builtin_amdgcn_fractf(tu);
+    const unsigned ub = (unsigned)(int)tu * esz;
     const float tv0 = fmaf(fmaf(pz0, mag, -vw.off_v), P.sv, P.cv);
     const float dtv = dz * mag * P.sv;
     float w = 1.0f;
     if (FBP) {
       const float U = (P.sad - qy) * P.inv_sad;
-      w = 1.0f / (U * U);
+      w = __builtin_amdgcn_rcpf(U * U);
     }
 #pragma unroll
     for (int t = 0; t < KZ; ++t) {
-      float tv = fmaf((float)t, dtv, tv0);
-      tv = fminf(fmaxf(tv, 0.0f), P.clv);
-      const float flv = floorf(tv);
-      const float av = tv - flv;
-      const unsigned o = (unsigned)flv * P.pnu;
-      const float a = ldp(row_base, o), b = ldp(row_base, o + 1);
-      const float c = ldp(row_base, o + P.pnu), d = ldp(row_base, o + P.pnu + 1);
+      const float tv = __builtin_amdgcn_fmed3f(fmaf((float)t, dtv, tv0), 0.0f, P.clv);
+      const float av = __builtin_amdgcn_fractf(tv);
+      const unsigned o = __umul24((unsigned)(int)tv, row_bytes) + ub;
+      const float a = bld<T>(rsrc, o, 0), b = bld<T>(rsrc, o + esz, 0);
+      const float c = bld<T>(rsrc, o, row_bytes), d = bld<T>(rsrc, o + esz, row_bytes);
       const float r0 = fmaf(au, b - a, a);
       const float r1 = fmaf(au, d - c, c);
       acc[t] = fmaf(w, fmaf(av, r1 - r0, r0), acc[t]);
     }
+    vw = nxt;
   }
 #pragma unroll
   for (int t = 0; t < KZ; ++t) {

@@ -211,13 +211,29 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return v;
 }
 
+// Raw buffer loads: 32-bit byte offsets, corner offsets in SGPRs, and zero
+// for any offset past the end of the buffer.
+template <typename T>
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  // the builtin returns the raw 32-bit word
+  return __uint_as_float((unsigned)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ float bload<unsigned short>(__amdgpu_buffer_rsrc_t r, unsigned voff,
+                                                       unsigned soff) {
+  return bf16_to_f32(__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0));
+}
+
 // One 64-lane workgroup per (detector column, row group, view).  RPL rays per
 // lane: rows j = row0 + r*64 + lane.
 template <typename T, int RPL>
 __global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restrict__ views,
                                                const T *__restrict__ vt, float *__restrict__ proj,
-                                               int accumulate, unsigned ngroups, unsigned nblocks) {
-  extern __shared__ float zl[];  // pz floats: one interpolated z-line
+                                               int accumulate, unsigned ngroups, unsigned nblocks,
+                                               unsigned vt_bytes) {
+  extern __shared__ float zl[];  // one interpolated z-line (whole 64-wide chunks)
   const unsigned L = xcd_remap(blockIdx.x, nblocks);
   const int i = (int)(L % (unsigned)P.nu);
   const unsigned rest = L / (unsigned)P.nu;
@@ -272,31 +288,40 @@ __global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restri
         pz[r] = pzl;
         qz[r] = qzl;
       }
-    const unsigned PZ = P.pz, PZX = P.pzx;
+    // Coordinates here are finite and the clamps keep them in [0, n+1], so
+    // med3 equals fminf(fmaxf()), truncation equals floor and fract(t)
+    // equals t - floor(t): same values as the per-ray loop.
+    const unsigned PZ = P.pz;
+    const unsigned esz = sizeof(T);
+    const unsigned so_x = PZ * esz, so_y = P.pzx * esz, so_xy = (P.pzx + PZ) * esz;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)vt, (short)0, (int)vt_bytes, 0x00020000);
     for (int m = 0; m < P.n; ++m) {
       const float fm = (float)m;
-      const float sx = clampf(fmaf(fm, qx, px0), P.cx);
-      const float sy = clampf(fmaf(fm, qy, py0), P.cy);
-      const float flx = floorf(sx), fly = floorf(sy);
-      const float ax = sx - flx, ay = sy - fly;
-      const int zlo = (int)floorf(clampf(fmaf(fm, qzl, pzl), P.cz));
-      const int zhi = (int)floorf(clampf(fmaf(fm, qzh, pzh), P.cz)) + 1;
-      const unsigned base = ((unsigned)fly * P.px + (unsigned)flx) * PZ;
-      for (int z = zlo + lane; z <= zhi; z += 64) {
-        const float v00 = ld(vt, base + z), v10 = ld(vt, base + PZ + z);
-        const float v01 = ld(vt, base + PZX + z), v11 = ld(vt, base + PZX + PZ + z);
+      const float sx = __builtin_amdgcn_fmed3f(fmaf(fm, qx, px0), 0.0f, P.cx);
+      const float sy = __builtin_amdgcn_fmed3f(fmaf(fm, qy, py0), 0.0f, P.cy);
+      const float ax = __builtin_amdgcn_fractf(sx), ay = __builtin_amdgcn_fractf(sy);
+      const int zlo = (int)__builtin_amdgcn_fmed3f(fmaf(fm, qzl, pzl), 0.0f, P.cz);
+      const int zhi = (int)__builtin_amdgcn_fmed3f(fmaf(fm, qzh, pzh), 0.0f, P.cz) + 1;
+      const int nch = (zhi - zlo + 64) >> 6;
+      const unsigned base = ((unsigned)(int)sy * P.px + (unsigned)(int)sx) * PZ + (unsigned)zlo;
+      for (int c = 0; c < nch; ++c) {
+        const unsigned zi = min(base + (unsigned)(lane + 64 * c), base - (unsigned)zlo + PZ - 1);
+        const unsigned voff = zi * esz;
+        const float v00 = bload<T>(rsrc, voff, 0), v10 = bload<T>(rsrc, voff, so_x);
+        const float v01 = bload<T>(rsrc, voff, so_y), v11 = bload<T>(rsrc, voff, so_xy);
         const float c0 = fmaf(ax, v10 - v00, v00);
         const float c1 = fmaf(ax, v11 - v01, v01);
-        zl[z - zlo] = fmaf(ay, c1 - c0, c0);
+        zl[lane + 64 * c] = fmaf(ay, c1 - c0, c0);
       }
       __syncthreads();
+      const float *zb = zl - zlo;
 #pragma unroll
       for (int r = 0; r < RPL; ++r) {
-        const float tz = clampf(fmaf(fm, qz[r], pz[r]), P.cz);
-        const float flz = floorf(tz);
-        const float az = tz - flz;
-        const int loc = (int)flz - zlo;
-        const float a = zl[loc], b = zl[loc + 1];
+        const float tz = __builtin_amdgcn_fmed3f(fmaf(fm, qz[r], pz[r]), 0.0f, P.cz);
+        const int iz = (int)tz;
+        const float az = __builtin_amdgcn_fractf(tz);
+        const float a = zb[iz], b = zb[iz + 1];
         acc[r] += fmaf(az, b - a, a);
       }
       __syncthreads();
@@ -462,22 +487,32 @@ int make_params(cbct_ctx ctx, const cbct_geom *g, float spp, FPParams &P) {
 
 template <typename T, int RPL>
 void launch_col(const FPParams &P, hipStream_t s, const ViewD *views, const T *vt, float *proj,
-                int acc, int nviews) {
+                int acc, int nviews, unsigned vt_bytes) {
   const unsigned ngroups = (unsigned)((P.nv + 64 * RPL - 1) / (64 * RPL));
   const unsigned nblocks = (unsigned)P.nu * ngroups * (unsigned)nviews;
-  hipLaunchKernelGGL((k_fp_col<T, RPL>), dim3(nblocks), dim3(64), P.pz * sizeof(float), s, P, views,
-                     vt, proj, acc, ngroups, nblocks);
+  const size_t lds = ((P.pz + 63) / 64) * 64 * sizeof(float);
+  hipLaunchKernelGGL((k_fp_col<T, RPL>), dim3(nblocks), dim3(64), lds, s, P, views, vt, proj, acc,
+                     ngroups, nblocks, vt_bytes);
 }
 
+// Rays per lane: whole 64-row blocks, at most 8 per lane (rows are split into
+// groups beyond that to keep registers, and so occupancy, in check).
 template <typename T>
 void dispatch_col(const FPParams &P, hipStream_t s, const ViewD *views, const T *vt, float *proj,
-                  int acc, int nviews) {
+                  int acc, int nviews, unsigned vt_bytes) {
   const int need = (P.nv + 63) / 64;
-  if (need <= 1) launch_col<T, 1>(P, s, views, vt, proj, acc, nviews);
-  else if (need <= 2) launch_col<T, 2>(P, s, views, vt, proj, acc, nviews);
-  else if (need <= 4) launch_col<T, 4>(P, s, views, vt, proj, acc, nviews);
-  else if (need <= 8) launch_col<T, 8>(P, s, views, vt, proj, acc, nviews);
-  else launch_col<T, 12>(P, s, views, vt, proj, acc, nviews);
+  const int groups = (need + 7) / 8;
+  const int rpl = (need + groups - 1) / groups;
+  switch (rpl) {
+    case 1: launch_col<T, 1>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 2: launch_col<T, 2>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 3: launch_col<T, 3>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 4: launch_col<T, 4>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 5: launch_col<T, 5>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 6: launch_col<T, 6>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 7: launch_col<T, 7>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    default: launch_col<T, 8>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+  }
 }
 
 template <typename T>
@@ -485,7 +520,9 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
            const T *vol, float *proj, int accumulate, hipStream_t s) {
   const size_t esz = sizeof(T);
   const ViewD *views = ctx->d_views_d + view0;
-  if (ctx->fp_engine == 0) {
+  // the column engine addresses the z-fastest copy with 32-bit byte offsets
+  const size_t vt_bytes = (size_t)P.pzx * padded_dim(g->ny) * esz;
+  if (ctx->fp_engine == 0 || vt_bytes + (size_t)P.pzx * esz * 2 >= (1ull << 31)) {
     const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
     int rc = cbct_reserve(ctx, ctx->vol_pad, npad * esz);
     if (rc) return rc;
@@ -511,7 +548,7 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
                      P.px, P.pz);
   CBCT_LAUNCH_CHECK(ctx, "k_pad_transpose");
   cbct_prof_mark(ctx, PROF_FP, 0, s);
-  dispatch_col<T>(P, s, views, vt, proj, accumulate, nviews);
+  dispatch_col<T>(P, s, views, vt, proj, accumulate, nviews, (unsigned)vt_bytes);
   cbct_prof_mark(ctx, PROF_FP, 1, s);
   CBCT_LAUNCH_CHECK(ctx, "k_fp_col");
   return CBCT_OK;
