@@ -44,10 +44,11 @@ struct cbct_ctx_s {
   int filter_engine = 0;
   int fp_engine = 1;             // 1: detector-column sharing, 0: one thread per ray
   float *d_h = nullptr;          // fp32 spatial kernel, 2*nu entries (h[|m|] mirrored)
-  unsigned short *d_thi = nullptr; // fp16 hi Toeplitz matrix, nu x nu (padded)
+  std::vector<double> h64;       // the same kernel in FP64, h[0..nu-1]
+  unsigned short *d_thi = nullptr; // fp16 hi Toeplitz matrix [np][kp] (built on first use)
   unsigned short *d_tlo = nullptr; // fp16 lo Toeplitz matrix
-  int toe_ld = 0;                // leading dimension of the fp16 matrices
-  float toe_scale = 1.0f;        // power-of-two scale applied to the fp16 matrix
+  int toe_ld = 0;                // kp: leading dimension of the fp16 matrices
+  float toe_inv_scale = 1.0f;    // inverse of the power-of-two scale applied to them
   // cosine weights cache
   float *d_cosw = nullptr;
   cbct_geom cosw_geom{};

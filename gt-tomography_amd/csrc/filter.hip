@@ -153,11 +153,245 @@ __global__ void __launch_bounds__(256) k_toeplitz_f32(const float *__restrict__ 
   }
 }
 
+// ------------------------------------------------- FP16 split MFMA engine
+// Y = X T with T[k][n] = h[|n-k|] (symmetric).  Both operands are split into
+// fp16 hi + lo parts after a power-of-two scaling (per row for X, one scale
+// for T), and three products hi*hi + hi*lo + lo*hi accumulate in FP32; the
+// dropped lo*lo term is below 2^-22 of |x||t| per product.
+//   X parts: [mp][kp] halves (mp = rows rounded up to 128, kp = nu rounded up
+//            to 32, zero padded), row scale inverse rinv[mp].
+//   T parts: [np][kp] halves, np = nu rounded up to 128 (rows n, contiguous
+//            over k, which for a symmetric T is also the transpose).
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int MB = 128, NB = 128, KB = 32;
+constexpr int LDK = KB + 8;  // LDS row pitch in halves (80 bytes: 16-byte aligned, staggered banks)
+
+// Map the linear block id so that XCD x (which receives blocks b == x mod 8)
+// processes the contiguous logical range [start_x, start_x + count_x): the
+// NB-column tiles of one row tile then share that XCD's L2 copy of the rows.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
+  const unsigned q = nblocks >> 3, rem = nblocks & 7u;
+  const unsigned x = b & 7u, idx = b >> 3;
+  return x * q + (x < rem ? x : rem) + idx;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One wave per row: weights (cosine plane, Parker), per-row power-of-two
+// scale putting max|x| in [2^13, 2^14), then the fp16 hi/lo split.
+__global__ void __launch_bounds__(256) k_split_rows(const float *__restrict__ in,
+                                                    const float *__restrict__ cosw, int nu, int nv,
+                                                    size_t nrows, size_t mp, int kp,
+                                                    const float *__restrict__ angles, int parker,
+                                                    float delta, float tan_delta,
+                                                    _Float16 *__restrict__ xh,
+                                                    _Float16 *__restrict__ xl,
+                                                    float *__restrict__ rinv) {
+  const size_t r = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= mp) return;
+  _Float16 *oh = xh + r * kp, *ol = xl + r * kp;
+  if (r >= nrows) {
+    for (int u = lane; u < kp; u += 64) oh[u] = ol[u] = (_Float16)0.0f;
+    if (lane == 0) rinv[r] = 0.0f;
+    return;
+  }
+  const float *x = in + r * nu;
+  const float *cw = cosw ? cosw + (r % nv) * nu : nullptr;
+  const float beta = parker ? angles[r / nv] * (3.14159265358979f / 180.0f) : 0.0f;
+  float mx = 0.0f;
+  for (int u = lane; u < nu; u += 64) {
+    float v = x[u];
+    if (cw) v *= cw[u];
+    if (parker) v *= parker_weight(u, nu, beta, delta, tan_delta);
+    mx = fmaxf(mx, fabsf(v));
+  }
+  mx = wave_max(mx);
+  float sc = 1.0f, inv = 1.0f;
+  if (mx > 0.0f && mx <= 3.4e38f) {
+    int ex;
+    (void)frexpf(mx, &ex);               // mx = f * 2^ex, f in [0.5, 1)
+    const int s = min(14 - ex, 126);
+    sc = ldexpf(1.0f, s);
+    inv = ldexpf(1.0f, -s);
+  }
+  for (int u = lane; u < kp; u += 64) {
+    float v = 0.0f;
+    if (u < nu) {
+      v = x[u];
+      if (cw) v *= cw[u];
+      if (parker) v *= parker_weight(u, nu, beta, delta, tan_delta);
+      v *= sc;
+    }
+    const _Float16 h = (_Float16)v;
+    oh[u] = h;
+    ol[u] = (_Float16)(v - (float)h);
+  }
+  if (lane == 0) rinv[r] = inv;
+}
+
+// 256 threads: four waves in a 2x2 arrangement, each owning a 64x64 output
+// block as 2x2 tiles of the 32x32x16 fp16 MFMA.  K advances in steps of 32
+// through LDS; the next step's global loads are issued before the MFMAs.
+__global__ void __launch_bounds__(256) k_toeplitz_mfma(const _Float16 *__restrict__ xh,
+                                                       const _Float16 *__restrict__ xl,
+                                                       const float *__restrict__ rinv,
+                                                       const _Float16 *__restrict__ th,
+                                                       const _Float16 *__restrict__ tl,
+                                                       float *__restrict__ Y, size_t nrows, int nu,
+                                                       int kp, unsigned ntn, unsigned nblocks,
+                                                       float inv_st) {
+  __shared__ __attribute__((aligned(16))) _Float16 sa[2][MB * LDK];
+  __shared__ __attribute__((aligned(16))) _Float16 sb[2][NB * LDK];
+  const unsigned L = xcd_remap(blockIdx.x, nblocks);
+  const unsigned tn = L % ntn, tm = L / ntn;
+  const size_t m0 = (size_t)tm * MB;
+  const int n0 = (int)tn * NB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  // loader: thread -> (row, 16-half segment) of the 128 x 32 tiles
+  const int lr = tid >> 1, lc = (tid & 1) * 16;
+  const uint4 *gah = reinterpret_cast<const uint4 *>(xh + (m0 + lr) * kp + lc);
+  const uint4 *gal = reinterpret_cast<const uint4 *>(xl + (m0 + lr) * kp + lc);
+  const uint4 *gbh = reinterpret_cast<const uint4 *>(th + (size_t)(n0 + lr) * kp + lc);
+  const uint4 *gbl = reinterpret_cast<const uint4 *>(tl + (size_t)(n0 + lr) * kp + lc);
+  uint4 ra[4], rb[4];
+  ra[0] = gah[0]; ra[1] = gah[1]; ra[2] = gal[0]; ra[3] = gal[1];
+  rb[0] = gbh[0]; rb[1] = gbh[1]; rb[2] = gbl[0]; rb[3] = gbl[1];
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.0f;
+
+  const int fr = lane & 31, fk = 8 * (lane >> 5);
+  for (int k0 = 0; k0 < kp; k0 += KB) {
+    __syncthreads();
+    uint4 *dah = reinterpret_cast<uint4 *>(&sa[0][lr * LDK + lc]);
+    uint4 *dal = reinterpret_cast<uint4 *>(&sa[1][lr * LDK + lc]);
+    uint4 *dbh = reinterpret_cast<uint4 *>(&sb[0][lr * LDK + lc]);
+    uint4 *dbl = reinterpret_cast<uint4 *>(&sb[1][lr * LDK + lc]);
+    dah[0] = ra[0]; dah[1] = ra[1]; dal[0] = ra[2]; dal[1] = ra[3];
+    dbh[0] = rb[0]; dbh[1] = rb[1]; dbl[0] = rb[2]; dbl[1] = rb[3];
+    __syncthreads();
+    if (k0 + KB < kp) {
+      const int o = (k0 + KB) / 8;  // uint4 = 8 halves
+      ra[0] = gah[o]; ra[1] = gah[o + 1]; ra[2] = gal[o]; ra[3] = gal[o + 1];
+      rb[0] = gbh[o]; rb[1] = gbh[o + 1]; rb[2] = gbl[o]; rb[3] = gbl[o + 1];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk += 16) {
+      half8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int ar = (wm * 64 + t * 32 + fr) * LDK + kk + fk;
+        const int br = (wn * 64 + t * 32 + fr) * LDK + kk + fk;
+        ah[t] = *reinterpret_cast<const half8 *>(&sa[0][ar]);
+        al[t] = *reinterpret_cast<const half8 *>(&sa[1][ar]);
+        bh[t] = *reinterpret_cast<const half8 *>(&sb[0][br]);
+        bl[t] = *reinterpret_cast<const half8 *>(&sb[1][br]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+    }
+  }
+  // D layout: lane holds column n = lane % 32, rows 8(i/4) + 4(lane/32) + i%4
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int n = n0 + wn * 64 + b * 32 + fr;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const size_t m = m0 + wm * 64 + a * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+        if (m < nrows && n < nu) Y[m * nu + n] = acc[a][b][i] * (rinv[m] * inv_st);
+      }
+    }
+}
+
 inline dim3 stride_grid(size_t n) {
   size_t b = (n + 255) / 256;
   if (b > 262144) b = 262144;
   if (b == 0) b = 1;
   return dim3((unsigned)b);
+}
+
+inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+// Host: dense fp16 hi/lo Toeplitz matrices from the FP64 kernel, built once
+// per filter on first use of the MFMA engine.
+int ensure_toeplitz_f16(cbct_ctx ctx) {
+  if (ctx->d_thi) return CBCT_OK;
+  const int nu = ctx->filter_nu;
+  const size_t kp = round_up(nu, KB), np = round_up(nu, NB);
+  double hmax = 0.0;
+  for (int m = 0; m < nu; ++m) hmax = std::fmax(hmax, std::fabs(ctx->h64[m]));
+  int ex = 0;
+  if (hmax > 0.0) (void)std::frexp(hmax, &ex);
+  const double sc = std::ldexp(1.0, 14 - ex);
+  std::vector<_Float16> hi(np * kp, (_Float16)0.0f), lo(np * kp, (_Float16)0.0f);
+  for (size_t n = 0; n < (size_t)nu; ++n)
+    for (size_t k = 0; k < (size_t)nu; ++k) {
+      const double v = ctx->h64[n > k ? n - k : k - n] * sc;
+      const _Float16 h = (_Float16)v;
+      hi[n * kp + k] = h;
+      lo[n * kp + k] = (_Float16)(v - (double)h);
+    }
+  CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_thi, sizeof(_Float16) * hi.size()));
+  CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_tlo, sizeof(_Float16) * lo.size()));
+  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_thi, hi.data(), sizeof(_Float16) * hi.size(), hipMemcpyHostToDevice));
+  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_tlo, lo.data(), sizeof(_Float16) * lo.size(), hipMemcpyHostToDevice));
+  ctx->toe_ld = (int)kp;
+  ctx->toe_inv_scale = (float)std::ldexp(1.0, ex - 14);
+  return CBCT_OK;
+}
+
+// Weighted (cosine plane, Parker) rows -> MFMA Toeplitz product into out.
+// The split stages the input in the workspace, so in may alias out.
+int run_toeplitz_mfma(cbct_ctx ctx, int nu, int nv, size_t nrows, const float *in, float *out,
+                      const float *cosw, const float *angles, int parker, float delta,
+                      float tan_delta, hipStream_t s) {
+  if (nrows == 0) return CBCT_OK;
+  int rc = ensure_toeplitz_f16(ctx);
+  if (rc) return rc;
+  const size_t kp = (size_t)ctx->toe_ld;
+  const size_t mp = round_up(nrows, MB);
+  const unsigned ntn = (unsigned)(round_up(nu, NB) / NB);
+  const size_t nblocks = mp / MB * ntn;
+  if (nblocks >= (1ull << 31) || mp / 4 >= (1ull << 31))
+    return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_ramp_filter: too many rows");
+  const size_t part = mp * kp * sizeof(_Float16);
+  rc = cbct_reserve(ctx, ctx->filt_tmp, 2 * part + mp * sizeof(float));
+  if (rc) return rc;
+  char *ws = static_cast<char *>(ctx->filt_tmp.ptr);
+  _Float16 *xh = reinterpret_cast<_Float16 *>(ws);
+  _Float16 *xl = reinterpret_cast<_Float16 *>(ws + part);
+  float *rinv = reinterpret_cast<float *>(ws + 2 * part);
+  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)(mp / 4)), dim3(256), 0, s, in, cosw, nu, nv,
+                     nrows, mp, (int)kp, angles, parker, delta, tan_delta, xh, xl, rinv);
+  CBCT_LAUNCH_CHECK(ctx, "k_split_rows");
+  cbct_prof_mark(ctx, PROF_FILTER, 0, s);
+  hipLaunchKernelGGL(k_toeplitz_mfma, dim3((unsigned)nblocks), dim3(256), 0, s, xh, xl, rinv,
+                     reinterpret_cast<const _Float16 *>(ctx->d_thi),
+                     reinterpret_cast<const _Float16 *>(ctx->d_tlo), out, nrows, nu, (int)kp, ntn,
+                     (unsigned)nblocks, ctx->toe_inv_scale);
+  cbct_prof_mark(ctx, PROF_FILTER, 1, s);
+  CBCT_LAUNCH_CHECK(ctx, "k_toeplitz_mfma");
+  return CBCT_OK;
 }
 
 // Host: one-sided response F[0..nu] -> spatial kernel h[0..nu-1] (FP64).
@@ -262,8 +496,12 @@ int cbct_filter_set(cbct_ctx ctx, int nu, const float *F) {
   for (int d = -(nu - 1); d <= nu - 1; ++d) hm[d + nu - 1] = (float)h[d < 0 ? -d : d];
   CBCT_HIP_CHECK(ctx, hipDeviceSynchronize());
   if (ctx->d_h) (void)hipFree(ctx->d_h);
+  if (ctx->d_thi) (void)hipFree(ctx->d_thi);
+  if (ctx->d_tlo) (void)hipFree(ctx->d_tlo);
   ctx->d_h = nullptr;
+  ctx->d_thi = ctx->d_tlo = nullptr;
   ctx->filter_nu = 0;
+  ctx->h64 = h;
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_h, sizeof(float) * hm.size()));
   CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_h, hm.data(), sizeof(float) * hm.size(), hipMemcpyHostToDevice));
   ctx->filter_nu = nu;
@@ -272,7 +510,7 @@ int cbct_filter_set(cbct_ctx ctx, int nu, const float *F) {
 
 int cbct_filter_engine(cbct_ctx ctx, int engine) {
   if (!ctx) return CBCT_ERR_ARG;
-  if (engine != 0)
+  if (engine != 0 && engine != 1)
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_filter_engine: engine %d not available", engine);
   ctx->filter_engine = engine;
   return CBCT_OK;
@@ -287,6 +525,9 @@ int cbct_ramp_filter(cbct_ctx ctx, int nu, size_t nrows, const float *in, float 
     return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_ramp_filter: filter was set for nu=%d, called with nu=%d", ctx->filter_nu, nu);
   int rc = cbct_set_device(ctx);
   if (rc) return rc;
+  if (ctx->filter_engine == 1)
+    return run_toeplitz_mfma(ctx, nu, 1, nrows, in, out, nullptr, nullptr, 0, 0.0f, 0.0f,
+                             as_stream(stream));
   return run_toeplitz(ctx, nu, nrows, in, out, as_stream(stream));
 }
 
@@ -313,6 +554,13 @@ int cbct_fdk_prefilter(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, 
   const float *d_angles = ctx->d_angles + view0;
   const float delta = std::atan(g->fov_u / (2.0f * g->sdd));
   const float tan_delta = std::tan(delta);
+  const size_t nrows = (size_t)g->nv * nviews;
+  if (ctx->filter_engine == 1) {
+    // weights fused into the fp16 split, then the MFMA product into out
+    rc = run_toeplitz_mfma(ctx, g->nu, g->nv, nrows, in, out, ctx->d_cosw, d_angles,
+                           short_scan ? 1 : 0, delta, tan_delta, s);
+    if (rc) return rc;
+  } else {
   // stage 1: weights into the staging buffer
   rc = cbct_reserve(ctx, ctx->filt_tmp, sizeof(float) * total);
   if (rc) return rc;
@@ -321,12 +569,12 @@ int cbct_fdk_prefilter(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, 
                      g->nv, nviews, d_angles, short_scan ? 1 : 0, delta, tan_delta);
   CBCT_LAUNCH_CHECK(ctx, "k_cos_parker");
   // stage 2: the Toeplitz filter from the staging buffer into out
-  const size_t nrows = (size_t)g->nv * nviews;
   dim3 grid(ceil_div(g->nu, TT), (unsigned)((nrows + TT - 1) / TT));
   cbct_prof_mark(ctx, PROF_FILTER, 0, s);
   hipLaunchKernelGGL(k_toeplitz_f32, grid, dim3(256), 0, s, tmp, out, ctx->d_h, g->nu, nrows);
   cbct_prof_mark(ctx, PROF_FILTER, 1, s);
   CBCT_LAUNCH_CHECK(ctx, "k_toeplitz_f32");
+  }
   // stage 3: full offset weights
   if (offset_correct) {
     hipLaunchKernelGGL(k_offset_weights, stride_grid(total), dim3(256), 0, s, out, g->nu, g->nv,

@@ -191,23 +191,54 @@ def test_fp_transpose_parity_and_adjoint(ctx):
     assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
 
 
+FILTER_ENGINES = [0, 1]   # 0: FP32 VALU Toeplitz, 1: FP16 hi/lo split MFMA
+
+
+@pytest.mark.parametrize("engine", FILTER_ENGINES)
 @pytest.mark.parametrize("nu", [48, 37, 256])
 @pytest.mark.parametrize("which", ["ramlak", "guo"])
-def test_ramp_filter_parity(ctx, nu, which):
-    print_case("filter", nu, which)
+def test_ramp_filter_parity(ctx, nu, which, engine):
+    print_case("filter", nu, which, "engine", engine)
     F = O.ramlak_filter(nu) if which == "ramlak" else O.guo_filter(nu)
     rows = np.random.default_rng(nu).standard_normal((70, nu))
     ref = O.filter_rows(rows, F.astype(np.float32))
     ctx.set_filter(F)
+    ctx.filter_engine(engine)
     out = torch.empty((70, nu), device="cuda")
     ctx.ramp_filter(nu, dev(rows), out)
     err = rel_l2(host(out), ref)
-    print("filter rel-L2", nu, which, err)
+    print("filter rel-L2", nu, which, "engine", engine, err)
     assert err <= TOL
 
 
+@pytest.mark.parametrize("engine", FILTER_ENGINES)
+def test_ramp_filter_dynamic_range(ctx, engine):
+    """Rows spanning 1e-6..1e8 (beyond the fp16 range), an all-zero row and a
+    single spike: each row must meet the tolerance on its own, which the
+    per-row power-of-two scaling of the split engine has to provide."""
+    nu, nr = 1024, 130          # 130 rows: one full 128-row tile plus a ragged one
+    print_case("filter dynamic range", nu, nr, "engine", engine)
+    rng = np.random.default_rng(11)
+    rows = rng.standard_normal((nr, nu)) * (10.0 ** rng.uniform(-6, 8, (nr, 1)))
+    rows[5] = 0.0
+    rows[7] = 0.0
+    rows[7, 300] = 3e7
+    F = O.guo_filter(nu)
+    ref = O.filter_rows(rows, F.astype(np.float32))
+    ctx.set_filter(F)
+    ctx.filter_engine(engine)
+    out = torch.empty((nr, nu), device="cuda")
+    ctx.ramp_filter(nu, dev(rows), out)
+    got = host(out)
+    assert np.all(got[5] == 0.0)
+    worst = max(rel_l2(got[r], ref[r]) for r in range(nr) if r != 5)
+    print("filter dynamic-range worst row rel-L2", engine, worst)
+    assert worst <= TOL
+
+
+@pytest.mark.parametrize("engine", FILTER_ENGINES)
 @pytest.mark.parametrize("short_scan,offset", [(False, False), (True, False), (False, True)])
-def test_fdk_prefilter_parity(ctx, short_scan, offset):
+def test_fdk_prefilter_parity(ctx, short_scan, offset, engine):
     g = geom(16, 16, 16, 40, 24, fov=(60.0, 36.0))
     n = 20
     arc = 200.0 if short_scan else 360.0
@@ -215,13 +246,14 @@ def test_fdk_prefilter_parity(ctx, short_scan, offset):
     offsets = np.zeros((n, 2), np.float32)
     if offset:
         offsets[:, 0] = 9.0
-    print_case("prefilter", short_scan, offset)
+    print_case("prefilter", short_scan, offset, "engine", engine)
     proj = np.random.default_rng(8).random(g.proj_shape(n))
     F = O.ramlak_filter(g.nu)
     ref = O.fdk_prefilter(to_oracle(g), angles, offsets, proj, F.astype(np.float32),
                           short_scan=short_scan, offset_correct=offset)
     ctx.set_views(angles, offsets)
     ctx.set_filter(F)
+    ctx.filter_engine(engine)
     pin = dev(proj)
     out = torch.empty_like(pin)
     ctx.fdk_prefilter(g, pin, out, short_scan=short_scan, offset_correct=offset)
