@@ -63,6 +63,98 @@ def cpu_baseline(cfg, vol_host, angles, budget_s=12.0):
                       "(FP32 OpenMP restatement; fp %.2fs, bp %.2fs)" % (n, tf, tb)}
 
 
+def measure_filter(ctx, g, proj, nviews, reps=3):
+    """FDK pre-filter (cosine weights + ramp filter) of the whole sinogram,
+    both engines.  Kernel time of the Toeplitz product from the library's own
+    HIP events; whole-call time (including the weighting / fp16 split pass)
+    from events on the same stream."""
+    import torch
+
+    from gtt import capi
+    ctx.set_filter(capi.default_filter(g.nu))
+    out = torch.empty_like(proj)
+    flops_alg = 2.0 * g.nu * g.nu * g.nv * nviews
+    res = {"rows": g.nv * nviews, "nu": g.nu, "flops_algorithmic": flops_alg}
+    for engine in (0, 1):
+        ctx.filter_engine(engine)
+        ctx.fdk_prefilter(g, proj, out)          # warm-up (builds workspaces)
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        kern, call = [], []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ctx.fdk_prefilter(g, proj, out)
+            e1.record()
+            torch.cuda.synchronize()
+            kern.append(ctx.profile_read()[2])
+            call.append(e0.elapsed_time(e1))
+        ctx.profile(False)
+        km = float(sorted(kern)[len(kern) // 2])
+        res["engine%d" % engine] = {
+            "kernel": "k_toeplitz_f32" if engine == 0 else "k_toeplitz_mfma",
+            "kernel_ms": round(km, 3), "call_ms": round(float(sorted(call)[len(call) // 2]), 3),
+            "tflops_algorithmic": round(flops_alg / (km / 1e3) / 1e12, 1)}
+    res["engine1"]["tflops_issued"] = round(3 * flops_alg / (res["engine1"]["kernel_ms"] / 1e3) / 1e12, 1)
+    res["engine1"]["mfma_frac_of_2500_tflops"] = round(res["engine1"]["tflops_issued"] / 2500.0, 4)
+    return res
+
+
+def serving_leg(ctx, lanes=4, n_batch=16, n_stream=24, load=0.8):
+    """Reconstruction latency and throughput, served (worker lanes on their
+    own streams) vs plain (one lane: requests strictly one after another).
+
+    batch:  uneven OS-SART scans (10 iterations, sizes drawn per scan), all
+            arriving at t=0.
+    stream: FDK (p=0.6) / 10-iteration OS-SART (p=0.4) at the same sizes,
+            Poisson arrivals at `load` x the plain server's capacity (rate from
+            a calibration pass of the same requests).
+    Served and plain results are compared bitwise."""
+    import numpy as np
+
+    from gtt import serving as S
+
+    def run(reqs, lanes_, arrivals=None):
+        for r in reqs:
+            r.volume = None
+        with S.Server(lanes=lanes_) as srv:
+            res, wall = S.run_stream(srv, reqs, arrivals)
+        return res, wall
+
+    def same(a, b):
+        return all(np.array_equal(x["volume"], y["volume"]) for x, y in zip(a, b))
+
+    out = {"lanes": lanes}
+    t0 = time.perf_counter()
+    reqs = [S.make_request(s, ctx) for s in S.uneven_scans(n_batch, seed=1003)]
+    gen_s = time.perf_counter() - t0
+    rp, wp = run(reqs, 1)
+    vp = [dict(volume=r["volume"].copy()) for r in rp]
+    rs, ws = run(reqs, lanes)
+    out["batch"] = {"plain": S.summarize(reqs, rp, wp), "served": S.summarize(reqs, rs, ws),
+                    "identical": same(vp, rs),
+                    "sizes": "%d OS-SART scans, edge %d..%d, %d..%d views" % (
+                        len(reqs), min(r.geom.nx for r in reqs), max(r.geom.nx for r in reqs),
+                        min(len(r.angles) for r in reqs), max(len(r.angles) for r in reqs))}
+    del reqs, rp, rs, vp
+    t0 = time.perf_counter()
+    reqs = [S.make_request(s, ctx) for s in S.uneven_scans(n_stream, seed=1004, p_fdk=0.6)]
+    gen_s += time.perf_counter() - t0
+    rc, _ = run(reqs, 1)
+    mean_service_s = float(np.mean([r["run_ms"] for r in rc])) / 1e3
+    rate = load / mean_service_s
+    arr = S.poisson_arrivals(len(reqs), rate, seed=7)
+    rp, wp = run(reqs, 1, arr)
+    vp = [dict(volume=r["volume"].copy()) for r in rp]
+    rs, ws = run(reqs, lanes, arr)
+    out["stream"] = {"rate_per_s": round(rate, 3), "load_vs_plain": load,
+                     "fdk": sum(r.kind == S.REQ_FDK for r in reqs),
+                     "plain": S.summarize(reqs, rp, wp), "served": S.summarize(reqs, rs, ws),
+                     "identical": same(vp, rs)}
+    out["request_generation_s"] = round(gen_s, 2)
+    return out
+
+
 def read_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
     if not os.path.exists(path):
@@ -84,9 +176,14 @@ def main():
                     help="storage type of the volume and sinogram (arithmetic is FP32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
-                    help="skip the extra bf16-storage measurement")
+                    help="skip the extra bf16-storage and filter measurements")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the served-vs-plain reconstruction latency measurement")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run this many FP+BP steps (for rocprof), no JSON extras")
+    ap.add_argument("--pmc-calibrate", action="store_true",
+                    help="with --profile-steps: first run a plain 1 GiB device copy whose "
+                         "FETCH_SIZE/WRITE_SIZE scale the projector kernels' counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -143,7 +240,19 @@ def main():
         return (t1 - t0) / steps * 1e3, float(np.mean(fp_ms)), float(np.mean(bp_ms)), proj, v_out
 
     if args.profile_steps:
-        measure(args.storage, args.profile_steps, 0)
+        if args.pmc_calibrate:
+            src = torch.ones(1 << 28, device="cuda")   # 1 GiB
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
+        _, _, _, proj, _ = measure(args.storage, args.profile_steps, 0)
+        # one FDK pre-filter of the whole sinogram per filter engine
+        ctx.set_filter(capi.default_filter(g.nu))
+        fout = torch.empty_like(proj)
+        for engine in (0, 1):
+            ctx.filter_engine(engine)
+            ctx.fdk_prefilter(g, proj, fout)
         torch.cuda.synchronize()
         print("profile run done", flush=True)
         return
@@ -176,6 +285,18 @@ def main():
                                  "bp_ms": round(b2, 3),
                                  "gups": round(2.0 * nvox * nviews / (ms2 / 1e3) / 1e9, 2)}
 
+    if not args.no_variants:
+        extra["fdk_prefilter"] = measure_filter(ctx, g, proj, nviews)
+
+    latency = None
+    if not args.no_serving:
+        del proj, v_out
+        torch.cuda.empty_cache()
+        try:
+            latency = serving_leg(ctx)
+        except Exception as e:  # reported in the line; the throughput figures stand
+            latency = {"error": "%s: %s" % (type(e).__name__, e)}
+
     cpu = None
     if not args.no_cpu_baseline:
         vol_host = vol.cpu().numpy()
@@ -201,6 +322,7 @@ def main():
                    "storage": args.storage, "samples_per_ray": nsamp},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "latency": latency,
         "detail": extra,
     }
     print(json.dumps(line), flush=True)

@@ -165,7 +165,7 @@ void cuConebeamProjectionOperator::mult_M(cuNDArray<float> *input, cuNDArray<flo
     if (binning_->bin(b).empty()) continue;
     const int nb = (int)angles_[b].size();
     check(cbct_fp(ctx_, &g, bin_view0_[b], nb, in_ptr, CBCT_F32, out_ptr, samples_per_pixel_,
-                  accumulate ? 1 : 0, nullptr), ctx_);
+                  accumulate ? 1 : 0, current_stream()), ctx_);
     // the volume pointer advances only past non-empty bins, as in the reference
     in_ptr += n3;
     out_ptr += plane * nb;
@@ -196,17 +196,17 @@ void cuConebeamProjectionOperator::mult_MH(cuNDArray<float> *input, cuNDArray<fl
       // full half-fan weights on a copy (conebeam_projection.cu:1242-1244)
       weighted = cuNDArray<float>(std::vector<size_t>{input->get_size(0), input->get_size(1), (size_t)nb});
       copy_d2d(weighted.get_data_ptr(), in_ptr, plane * nb * sizeof(float));
-      check(cbct_offset_weights(ctx_, &g, bin_view0_[b], nb, weighted.get_data_ptr(), 0, nullptr), ctx_);
+      check(cbct_offset_weights(ctx_, &g, bin_view0_[b], nb, weighted.get_data_ptr(), 0, current_stream()), ctx_);
       src = weighted.get_data_ptr();
     }
-    check(cbct_bp(ctx_, &g, bin_view0_[b], nb, src, CBCT_F32, out_ptr, 0, accumulate ? 1 : 0, nullptr), ctx_);
+    check(cbct_bp(ctx_, &g, bin_view0_[b], nb, src, CBCT_F32, out_ptr, 0, accumulate ? 1 : 0, current_stream()), ctx_);
     in_ptr += plane * nb;
     out_ptr += n3;
   }
   if (mask_) {
     cbct_ctx c = device_context();
     check(cbct_apply_mask(c, output->get_number_of_elements(), output->get_data_ptr(),
-                          mask_->get_data_ptr(), nullptr), c);
+                          mask_->get_data_ptr(), current_stream()), c);
   }
 }
 
@@ -217,7 +217,7 @@ void cuConebeamProjectionOperator::offset_correct(cuNDArray<float> *projections)
   if ((int)nv > (int)acquisition_->get_geometry()->get_angles().size())
     throw std::runtime_error("Error: offset_correct: inconsistent sizes of input arrays/vectors");
   const cbct_geom g = geom_for(1, 1, 1, projections->get_size(0), projections->get_size(1));
-  check(cbct_offset_weights(ctx_, &g, all_view0_, (int)nv, projections->get_data_ptr(), 1, nullptr), ctx_);
+  check(cbct_offset_weights(ctx_, &g, all_view0_, (int)nv, projections->get_data_ptr(), 1, current_stream()), ctx_);
 }
 
 std::shared_ptr<cuNDArray<unsigned char>> cuConebeamProjectionOperator::calculate_mask(
@@ -234,7 +234,7 @@ std::shared_ptr<cuNDArray<unsigned char>> cuConebeamProjectionOperator::calculat
   for (unsigned int b = 0; b < binning_->get_number_of_bins(); ++b) {
     if (binning_->bin(b).empty()) continue;
     const int nb = (int)angles_[b].size();
-    check(cbct_spacecarver(ctx_, &g, bin_view0_[b], nb, ip, mp, limit, nullptr), ctx_);
+    check(cbct_spacecarver(ctx_, &g, bin_view0_[b], nb, ip, mp, limit, current_stream()), ctx_);
     ip += plane * nb;
     mp += prod(dims3d);
   }
@@ -366,7 +366,7 @@ void cuTVPrimalDualOperator::primalDual(cuNDArray<float> *in, cuNDArray<float> *
   cbct_ctx c = device_context();
   check(cbct_tv_primal_dual(c, (int)in->get_size(0), (int)in->get_size(1), (int)in->get_size(2),
                             (int)remaining, in->get_data_ptr(), out->get_data_ptr(), sigma, alpha,
-                            weight, accumulate ? 1 : 0, nullptr), c);
+                            weight, accumulate ? 1 : 0, current_stream()), c);
 }
 
 template <>
@@ -375,7 +375,7 @@ void osMOMSolverD<cuNDArray<float>>::updateG(cuNDArray<float> &g, cuNDArray<floa
                                              float tau, float avg_lambda) {
   cbct_ctx c = device_context();
   check(cbct_update_g(c, x.get_number_of_elements(), g.get_data_ptr(), x.get_data_ptr(),
-                      s.get_data_ptr(), precon.get_data_ptr(), tau, avg_lambda, nullptr), c);
+                      s.get_data_ptr(), precon.get_data_ptr(), tau, avg_lambda, current_stream()), c);
 }
 
 template <>
@@ -390,7 +390,7 @@ void osMOMSolverD<cuNDArray<float>>::fused_tv_update(cuTVPrimalDualOperator *tv,
   check(cbct_tv_update_g(c, (int)x->get_size(0), (int)x->get_size(1), (int)x->get_size(2),
                          (int)remaining, x->get_data_ptr(), g->get_data_ptr(), s.get_data_ptr(),
                          precon.get_data_ptr(), sigma, tv->get_alpha(), tv->get_weight(), tau,
-                         avg_lambda, nullptr), c);
+                         avg_lambda, current_stream()), c);
   x->swap(*g);
 }
 

@@ -142,7 +142,7 @@ void hoCuConebeamProjectionOperator::mult_M(hoCuNDArray<float> *image,
     cuNDArray<float> dvol(vol_view);
     cuNDArray<float> dproj(std::vector<size_t>{nu, nv, idx.size()});
     check(cbct_fp(ctx_, &g, bin_view0_[b], (int)idx.size(), dvol.get_data_ptr(), CBCT_F32,
-                  dproj.get_data_ptr(), samples_per_pixel_, 0, nullptr), ctx_);
+                  dproj.get_data_ptr(), samples_per_pixel_, 0, current_stream()), ctx_);
     hoCuNDArray<float> hproj(dproj.dims());
     dproj.to_host(&hproj);
     for (size_t p = 0; p < idx.size(); ++p)
@@ -211,15 +211,15 @@ void hoCuConebeamProjectionOperator::mult_MH(hoCuNDArray<float> *projections,
     if (accumulate) copy_h2d(dimg.get_data_ptr(), img3, n3 * sizeof(float));
     if (use_fbp_) {
       check(cbct_fdk_prefilter(ctx_, &g, bin_view0_[b], nb, dproj.get_data_ptr(), dproj.get_data_ptr(),
-                               short_scan_ ? 1 : 0, use_offset_correction_ ? 1 : 0, nullptr), ctx_);
+                               short_scan_ ? 1 : 0, use_offset_correction_ ? 1 : 0, current_stream()), ctx_);
       check(cbct_bp(ctx_, &g, bin_view0_[b], nb, dproj.get_data_ptr(), CBCT_F32, dimg.get_data_ptr(), 1,
-                    accumulate ? 1 : 0, nullptr), ctx_);
+                    accumulate ? 1 : 0, current_stream()), ctx_);
       dimg *= 2.0f / float(nb);  // .cpp:253
     } else {
       if (use_offset_correction_)
-        check(cbct_offset_weights(ctx_, &g, bin_view0_[b], nb, dproj.get_data_ptr(), 1, nullptr), ctx_);
+        check(cbct_offset_weights(ctx_, &g, bin_view0_[b], nb, dproj.get_data_ptr(), 1, current_stream()), ctx_);
       check(cbct_bp(ctx_, &g, bin_view0_[b], nb, dproj.get_data_ptr(), CBCT_F32, dimg.get_data_ptr(), 0,
-                    accumulate ? 1 : 0, nullptr), ctx_);
+                    accumulate ? 1 : 0, current_stream()), ctx_);
     }
     copy_d2h(img3, dimg.get_data_ptr(), n3 * sizeof(float));
   }
@@ -234,7 +234,7 @@ void hoCuConebeamProjectionOperator::offset_correct(hoCuNDArray<float> *projecti
     throw std::runtime_error("Error: offset_correct: inconsistent sizes of input arrays/vectors");
   cuNDArray<float> d(*projections);
   const cbct_geom g = geom_for(1, 1, 1, projections->get_size(0), projections->get_size(1));
-  check(cbct_offset_weights(ctx_, &g, 0, (int)nv, d.get_data_ptr(), 1, nullptr), ctx_);
+  check(cbct_offset_weights(ctx_, &g, 0, (int)nv, d.get_data_ptr(), 1, current_stream()), ctx_);
   d.to_host(projections);
 }
 

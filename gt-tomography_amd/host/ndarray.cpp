@@ -5,14 +5,16 @@
 #include <hip/hip_runtime_api.h>
 
 #include <map>
-#include <mutex>
 #include <string>
 
 namespace gtt {
 
 namespace {
-std::mutex g_mu;
-std::map<int, cbct_ctx> g_ctx;
+// Contexts of the calling thread, by device.  Not destroyed at thread exit
+// (the HIP runtime may already be gone at process exit); worker threads call
+// release_thread_resources().
+thread_local std::map<int, cbct_ctx> t_ctx;
+thread_local hipStream_t t_stream = nullptr;
 
 void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -22,14 +24,20 @@ void hip_check(hipError_t e, const char *what) {
 cbct_ctx device_context() {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
-  std::lock_guard<std::mutex> lk(g_mu);
-  auto it = g_ctx.find(dev);
-  if (it != g_ctx.end()) return it->second;
+  auto it = t_ctx.find(dev);
+  if (it != t_ctx.end()) return it->second;
   cbct_ctx c = nullptr;
   int rc = cbct_ctx_create(dev, &c);
   if (rc) throw std::runtime_error(cbct_last_error(nullptr));
-  g_ctx[dev] = c;
+  t_ctx[dev] = c;
   return c;
+}
+
+void *current_stream() { return t_stream; }
+void set_current_stream(void *stream) { t_stream = static_cast<hipStream_t>(stream); }
+void release_thread_resources() {
+  for (auto &kv : t_ctx) cbct_ctx_destroy(kv.second);
+  t_ctx.clear();
 }
 
 void check(int rc, cbct_ctx ctx) {
@@ -39,30 +47,39 @@ void check(int rc, cbct_ctx ctx) {
 void device_malloc(void **p, size_t bytes) { hip_check(hipMalloc(p, bytes), "hipMalloc"); }
 void device_free(void *p) { (void)hipFree(p); }
 void copy_d2d(void *dst, const void *src, size_t bytes) {
-  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, nullptr), "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, t_stream), "hipMemcpyAsync");
 }
+// Host copies are ordered after the thread's queued work and complete before
+// returning (the host buffer may be reused or read right away).
 void copy_h2d(void *dst, const void *src, size_t bytes) {
-  hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, t_stream), "hipMemcpyAsync H2D");
+  hip_check(hipStreamSynchronize(t_stream), "hipStreamSynchronize");
 }
 void copy_d2h(void *dst, const void *src, size_t bytes) {
-  hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, t_stream), "hipMemcpyAsync D2H");
+  hip_check(hipStreamSynchronize(t_stream), "hipStreamSynchronize");
 }
-void device_synchronize() { hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+void device_synchronize() {
+  if (t_stream)
+    hip_check(hipStreamSynchronize(t_stream), "hipStreamSynchronize");
+  else
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
 
 #define CTX device_context()
 
 void clear(cuNDArray<float> *x) { fill(x, 0.0f); }
 void fill(cuNDArray<float> *x, float v) {
   cbct_ctx c = CTX;
-  check(cbct_fill(c, x->get_number_of_elements(), x->get_data_ptr(), v, nullptr), c);
+  check(cbct_fill(c, x->get_number_of_elements(), x->get_data_ptr(), v, current_stream()), c);
 }
 void clamp_min(cuNDArray<float> *x, float v) {
   cbct_ctx c = CTX;
-  check(cbct_clamp_min(c, x->get_number_of_elements(), x->get_data_ptr(), v, nullptr), c);
+  check(cbct_clamp_min(c, x->get_number_of_elements(), x->get_data_ptr(), v, current_stream()), c);
 }
 void reciprocal_inplace(cuNDArray<float> *x) {
   cbct_ctx c = CTX;
-  check(cbct_reciprocal(c, x->get_number_of_elements(), x->get_data_ptr(), nullptr), c);
+  check(cbct_reciprocal(c, x->get_number_of_elements(), x->get_data_ptr(), current_stream()), c);
 }
 static void same_size(const cuNDArray<float> *a, const cuNDArray<float> *b, const char *who) {
   if (a->get_number_of_elements() != b->get_number_of_elements())
@@ -71,25 +88,25 @@ static void same_size(const cuNDArray<float> *a, const cuNDArray<float> *b, cons
 void axpy(float a, const cuNDArray<float> *x, cuNDArray<float> *y) {
   same_size(x, y, "axpy");
   cbct_ctx c = CTX;
-  check(cbct_axpy(c, y->get_number_of_elements(), a, x->get_data_ptr(), y->get_data_ptr(), nullptr), c);
+  check(cbct_axpy(c, y->get_number_of_elements(), a, x->get_data_ptr(), y->get_data_ptr(), current_stream()), c);
 }
 double dot(const cuNDArray<float> *a, const cuNDArray<float> *b) {
   same_size(a, b, "dot");
   cbct_ctx c = CTX;
   double r = 0;
-  check(cbct_reduce_f64(c, CBCT_DOT, a->get_number_of_elements(), a->get_data_ptr(), b->get_data_ptr(), &r, nullptr), c);
+  check(cbct_reduce_f64(c, CBCT_DOT, a->get_number_of_elements(), a->get_data_ptr(), b->get_data_ptr(), &r, current_stream()), c);
   return r;
 }
 double nrm2(const cuNDArray<float> *a) {
   cbct_ctx c = CTX;
   double r = 0;
-  check(cbct_reduce_f64(c, CBCT_NRM2, a->get_number_of_elements(), a->get_data_ptr(), nullptr, &r, nullptr), c);
+  check(cbct_reduce_f64(c, CBCT_NRM2, a->get_number_of_elements(), a->get_data_ptr(), nullptr, &r, current_stream()), c);
   return r;
 }
 double asum(const cuNDArray<float> *a) {
   cbct_ctx c = CTX;
   double r = 0;
-  check(cbct_reduce_f64(c, CBCT_ASUM, a->get_number_of_elements(), a->get_data_ptr(), nullptr, &r, nullptr), c);
+  check(cbct_reduce_f64(c, CBCT_ASUM, a->get_number_of_elements(), a->get_data_ptr(), nullptr, &r, current_stream()), c);
   return r;
 }
 
@@ -98,7 +115,7 @@ void sart_residual(cuNDArray<float> *r, const cuNDArray<float> *b, const cuNDArr
   same_size(r, w, "sart_residual");
   cbct_ctx c = CTX;
   check(cbct_sart_residual(c, r->get_number_of_elements(), r->get_data_ptr(), b->get_data_ptr(),
-                           w->get_data_ptr(), nullptr), c);
+                           w->get_data_ptr(), current_stream()), c);
 }
 void sart_update(cuNDArray<float> *x, const cuNDArray<float> *img, const cuNDArray<float> *wx,
                  float step, bool nonneg) {
@@ -106,7 +123,7 @@ void sart_update(cuNDArray<float> *x, const cuNDArray<float> *img, const cuNDArr
   same_size(x, wx, "sart_update");
   cbct_ctx c = CTX;
   check(cbct_sart_update(c, x->get_number_of_elements(), x->get_data_ptr(), img->get_data_ptr(),
-                         wx->get_data_ptr(), step, nonneg ? 1 : 0, nullptr), c);
+                         wx->get_data_ptr(), step, nonneg ? 1 : 0, current_stream()), c);
 }
 void mom_gradient_step(cuNDArray<float> *z, const cuNDArray<float> *img,
                        const cuNDArray<float> *precon, float step) {
@@ -114,7 +131,7 @@ void mom_gradient_step(cuNDArray<float> *z, const cuNDArray<float> *img,
   same_size(z, precon, "mom_gradient_step");
   cbct_ctx c = CTX;
   check(cbct_mom_gradient_step(c, z->get_number_of_elements(), z->get_data_ptr(), img->get_data_ptr(),
-                               precon->get_data_ptr(), step, nullptr), c);
+                               precon->get_data_ptr(), step, current_stream()), c);
 }
 void mom_momentum(cuNDArray<float> *x, const cuNDArray<float> *xold, cuNDArray<float> *z, float cc,
                   bool nonneg) {
@@ -122,47 +139,47 @@ void mom_momentum(cuNDArray<float> *x, const cuNDArray<float> *xold, cuNDArray<f
   same_size(x, z, "mom_momentum");
   cbct_ctx c = CTX;
   check(cbct_mom_momentum(c, x->get_number_of_elements(), x->get_data_ptr(), xold->get_data_ptr(),
-                          z->get_data_ptr(), cc, nonneg ? 1 : 0, nullptr), c);
+                          z->get_data_ptr(), cc, nonneg ? 1 : 0, current_stream()), c);
 }
 
 template <>
 cuNDArray<float> &cuNDArray<float>::operator*=(float a) {
   cbct_ctx c = CTX;
-  check(cbct_scale(c, get_number_of_elements(), data_, a, nullptr), c);
+  check(cbct_scale(c, get_number_of_elements(), data_, a, current_stream()), c);
   return *this;
 }
 template <>
 cuNDArray<float> &cuNDArray<float>::operator+=(float a) {
   cbct_ctx c = CTX;
-  check(cbct_add_scalar(c, get_number_of_elements(), data_, a, nullptr), c);
+  check(cbct_add_scalar(c, get_number_of_elements(), data_, a, current_stream()), c);
   return *this;
 }
 template <>
 cuNDArray<float> &cuNDArray<float>::operator*=(const cuNDArray<float> &a) {
   same_size(this, &a, "operator*=");
   cbct_ctx c = CTX;
-  check(cbct_mul(c, get_number_of_elements(), a.data_, data_, nullptr), c);
+  check(cbct_mul(c, get_number_of_elements(), a.data_, data_, current_stream()), c);
   return *this;
 }
 template <>
 cuNDArray<float> &cuNDArray<float>::operator/=(const cuNDArray<float> &a) {
   same_size(this, &a, "operator/=");
   cbct_ctx c = CTX;
-  check(cbct_div(c, get_number_of_elements(), a.data_, data_, nullptr), c);
+  check(cbct_div(c, get_number_of_elements(), a.data_, data_, current_stream()), c);
   return *this;
 }
 template <>
 cuNDArray<float> &cuNDArray<float>::operator+=(const cuNDArray<float> &a) {
   same_size(this, &a, "operator+=");
   cbct_ctx c = CTX;
-  check(cbct_add(c, get_number_of_elements(), a.data_, data_, nullptr), c);
+  check(cbct_add(c, get_number_of_elements(), a.data_, data_, current_stream()), c);
   return *this;
 }
 template <>
 cuNDArray<float> &cuNDArray<float>::operator-=(const cuNDArray<float> &a) {
   same_size(this, &a, "operator-=");
   cbct_ctx c = CTX;
-  check(cbct_sub(c, get_number_of_elements(), a.data_, data_, nullptr), c);
+  check(cbct_sub(c, get_number_of_elements(), a.data_, data_, current_stream()), c);
   return *this;
 }
 

@@ -2,8 +2,9 @@
 // reference's operators and solvers use (first index fastest, dimension
 // vectors, views onto foreign memory, deep copy on copy-construction) and the
 // element-wise / BLAS free functions of cuNDArray_math / cuNDArray_blas.
-// Device arithmetic runs through libcbct (include/cbct.h) on the default HIP
-// stream of the current thread, like the reference's legacy default stream.
+// Device arithmetic runs through libcbct (include/cbct.h) on the calling
+// thread's current HIP stream: the default stream (like the reference's
+// legacy default stream) unless a worker thread selected its own.
 #pragma once
 
 #include <cstddef>
@@ -17,8 +18,16 @@
 
 namespace gtt {
 
-// Per-device context used by the array math (one per process and device).
+// Per-device context used by the array math (one per thread and device, so
+// that threads running on different streams share no reduction workspace).
 cbct_ctx device_context();
+// HIP stream (a hipStream_t) that the calling thread's array math, copies and
+// operator launches go to; nullptr is the default stream.  Copies between
+// host and device synchronise this stream only.
+void *current_stream();
+void set_current_stream(void *stream);
+// Destroy the calling thread's contexts (worker threads, before they exit).
+void release_thread_resources();
 // HIP device current on the calling thread.
 int current_device();
 // Throw std::runtime_error with the library's message when rc != 0.

@@ -58,6 +58,55 @@ int gtt_subset_op_views(gtt_subset_op op, int subset);
 int gtt_subset_op_mult_M(gtt_subset_op op, float *vol_dev, float *proj_dev, int subset, int accumulate);
 int gtt_subset_op_mult_MH(gtt_subset_op op, float *proj_dev, float *vol_dev, int subset, int accumulate);
 
+/* ------------------------------------------------------------------------
+ * Reconstruction server: FDK and OS-SART requests of any size served by
+ * worker lanes on one GPU.  No counterpart in the reference, which runs one
+ * reconstruction per process (xray/FDK_reconstruct_3d.cpp:24-207,
+ * xray/cuCBOS_reconstruct.cpp:126-579); each request runs exactly those
+ * programs (gtt_fdk_reconstruct_3d / gtt_os_sart with one bin), on its
+ * lane's own HIP stream.  Results do not depend on the number of lanes or on
+ * what else is running.
+ *
+ * Buffers in a request are caller-owned and must stay valid until
+ * gtt_server_wait() returns for it.  Admission: the head of the FIFO starts
+ * once a lane is free and the estimated device footprints of the running
+ * requests plus its own (gtt_request_footprint) fit the budget given at
+ * creation (0: three quarters of the free device memory).
+ */
+enum { GTT_REQ_FDK = 0, GTT_REQ_OS_SART = 1 };
+
+typedef struct {
+  int kind;                   /* GTT_REQ_FDK or GTT_REQ_OS_SART */
+  cbct_geom geom;
+  int nviews;
+  const float *angles_deg;    /* nviews */
+  const float *offsets;       /* 2*nviews (u, v) or NULL */
+  const float *projections;   /* host, nu x nv x nviews (u fastest) */
+  const float *filter;        /* FDK: nu+1 taps, or NULL for the default filter */
+  float samples_per_pixel;    /* FDK rescale projection; <= 0 selects 1.5 */
+  int subsets, iterations;    /* OS-SART */
+  float beta, gamma;
+  int nonneg;
+  float *volume_out;          /* host, nx x ny x nz */
+} gtt_request;
+
+typedef struct {
+  int status;                 /* 0, or -1 with message */
+  double queue_ms;            /* submit -> start on a lane */
+  double run_ms;              /* start -> result in volume_out */
+  double latency_ms;          /* submit -> result */
+  double scale;               /* FDK least-squares rescale factor */
+  char message[256];
+} gtt_result;
+
+typedef struct gtt_server_s *gtt_server;
+int gtt_server_create(int device, int lanes, size_t hbm_budget_bytes, gtt_server *out);
+int gtt_server_destroy(gtt_server s);  /* drains the queue first */
+int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id);
+int gtt_server_wait(gtt_server s, long long id, gtt_result *res);
+const char *gtt_server_last_error(gtt_server s);
+size_t gtt_request_footprint(const gtt_request *req);
+
 #ifdef __cplusplus
 }
 #endif

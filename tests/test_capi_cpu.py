@@ -62,3 +62,38 @@ def test_declared_entry_points_cite_reference():
     assert "conebeam_backwards_projection" in txt
     assert "cuTVPrimalDualOperator" in txt
     assert "osMOMSolverD" in txt and "osSARTSolver" in txt
+
+
+def test_libgtt_exports_every_declared_symbol():
+    import re
+
+    from gtt import operators as ops
+    hdr = os.path.join(os.path.dirname(capi.HEADER), "gtt_c.h")
+    declared = sorted(set(re.findall(r"\b(gtt_\w+)\s*\(", open(hdr).read())))
+    assert "gtt_server_submit" in declared and len(declared) >= 15
+    out = subprocess.run(["nm", "-D", "--defined-only", ops.lib_path()], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+
+
+def test_request_footprint_and_server_without_gpu():
+    """Host-only parts of the server API: the footprint estimate (volume- and
+    padded-projection-sized arrays, plus slack) and a clean error from
+    gtt_server_create when no device can be selected."""
+    from gtt import serving as S
+    spec = S.ScanSpec(n=40, nviews=30, nu=64, nv=48, subsets=5, kind=S.REQ_OS_SART, seed=1)
+    g = spec.geom()
+    req = S.Request(kind=S.REQ_OS_SART, geom=g, angles=spec.angles(),
+                    projections=np.zeros(g.proj_shape(30), np.float32))
+    nvox, nproj = 40 ** 3, (64 + 3) * (48 + 3) * 30
+    assert req.footprint() == 4 * (6 * nvox + 5 * nproj) + (64 << 20)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present")
+    except Exception:
+        pass
+    with pytest.raises(RuntimeError, match="gtt_server_create"):
+        S.Server(lanes=2)
