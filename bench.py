@@ -3,9 +3,12 @@
 
 Workload (default): 512^3 volume at 0.5 mm, 720 views over 360 degrees,
 1024x768 flat detector at 0.39 mm, SAD 1000 mm, SDD 1500 mm, seeded
-ellipsoid phantom (synthetic).  One step = one forward projection of the
-volume onto all 720 views + one backprojection of all 720 views, inputs
-resident in HBM.  GUPS = 2 * Nvox * Nviews / t_step / 1e9.
+ellipsoid phantom (synthetic); the projectors' working copies of the volume
+and the sinogram in BF16, arithmetic in FP32 (--storage f32 for FP32
+copies).  One step = one forward projection of the volume onto all 720
+views + one backprojection of all 720 views, inputs resident in HBM.
+GUPS = 2 * Nvox * Nviews / t_step / 1e9.  The plain FP32 path is timed in
+the same run, with the error of each BF16 stage against it.
 
 Prints one JSON line (rank 0) with the roofline of the dominant kernel and
 a CPU baseline (FP32 OpenMP restatement timed on this host on a bounded
@@ -75,6 +78,7 @@ def measure_filter(ctx, g, proj, nviews, reps=3):
     out = torch.empty_like(proj)
     flops_alg = 2.0 * g.nu * g.nu * g.nv * nviews
     res = {"rows": g.nv * nviews, "nu": g.nu, "flops_algorithmic": flops_alg}
+    out0 = None
     for engine in (0, 1):
         ctx.filter_engine(engine)
         ctx.fdk_prefilter(g, proj, out)          # warm-up (builds workspaces)
@@ -90,6 +94,8 @@ def measure_filter(ctx, g, proj, nviews, reps=3):
             kern.append(ctx.profile_read()[2])
             call.append(e0.elapsed_time(e1))
         ctx.profile(False)
+        if engine == 0:
+            out0 = out.clone()
         km = float(sorted(kern)[len(kern) // 2])
         res["engine%d" % engine] = {
             "kernel": "k_toeplitz_f32" if engine == 0 else "k_toeplitz_mfma",
@@ -97,6 +103,10 @@ def measure_filter(ctx, g, proj, nviews, reps=3):
             "tflops_algorithmic": round(flops_alg / (km / 1e3) / 1e12, 1)}
     res["engine1"]["tflops_issued"] = round(3 * flops_alg / (res["engine1"]["kernel_ms"] / 1e3) / 1e12, 1)
     res["engine1"]["mfma_frac_of_2500_tflops"] = round(res["engine1"]["tflops_issued"] / 2500.0, 4)
+    # the split-precision MFMA result against the FP32 engine (itself ~2e-7
+    # from the FP64 oracle in the parity tests), FP64 device reductions
+    res["engine1"]["rel_l2_vs_engine0"] = ctx.nrm2(out - out0) / ctx.nrm2(out0)
+    ctx.filter_engine(1)
     return res
 
 
@@ -172,8 +182,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="prec512")
-    ap.add_argument("--storage", choices=["f32", "bf16"], default="f32",
-                    help="storage type of the volume and sinogram (arithmetic is FP32)")
+    ap.add_argument("--storage", choices=["f32", "bf16"], default="bf16",
+                    help="storage type of the projectors' working copies of the volume and "
+                         "sinogram (arithmetic is FP32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the extra bf16-storage and filter measurements")
@@ -211,17 +222,17 @@ def main():
     torch.cuda.synchronize()
 
     def measure(storage, steps, warmup):
-        dt = torch.float32 if storage == "f32" else torch.bfloat16
-        v_in = vol.to(dt).contiguous()
+        """One step = FP of the volume onto all views + BP of all views.  With
+        'bf16' the projectors' working copies of the volume and the sinogram
+        (the arrays their kernels gather from) are BF16, converted inside the
+        copy kernels each call; arithmetic and the caller's arrays are FP32."""
+        ctx.storage(storage)
         proj = torch.empty(g.proj_shape(nviews), device="cuda", dtype=torch.float32)
-        proj_in = proj if storage == "f32" else torch.empty(proj.shape, device="cuda", dtype=dt)
         v_out = torch.empty_like(vol)
 
         def step():
-            ctx.fp(g, v_in, proj)
-            if storage != "f32":
-                proj_in.copy_(proj)
-            ctx.bp(g, proj_in, v_out)
+            ctx.fp(g, vol, proj)
+            ctx.bp(g, proj, v_out)
 
         for _ in range(warmup):
             step()
@@ -237,7 +248,12 @@ def main():
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         ctx.profile(False)
+        ctx.storage("f32")
         return (t1 - t0) / steps * 1e3, float(np.mean(fp_ms)), float(np.mean(bp_ms)), proj, v_out
+
+    def rel_err(a, b):
+        """||a - b|| / ||b|| with FP64 device reductions."""
+        return ctx.nrm2(a - b) / ctx.nrm2(b)
 
     if args.profile_steps:
         if args.pmc_calibrate:
@@ -267,23 +283,39 @@ def main():
     kms = fp_ms if kind == "fp" else bp_ms
     ab = algorithmic_bytes(kind, nvox, nviews, nrays, nsamp, s_vol, s_vol)
     achieved = ab / (kms / 1e3) / 1e9
+    kname = "k_fp_col" if kind == "fp" else "k_bp"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic("k_fp_col" if kind == "fp" else "k_bp", cfg.name),
-                "kernel": "k_fp_col" if kind == "fp" else "k_bp",
-                "kernel_ms": round(kms, 3), "algorithmic_bytes": ab,
+                "traffic": read_traffic(kname, "%s_%s" % (cfg.name, args.storage)),
+                "kernel": kname, "kernel_ms": round(kms, 3), "algorithmic_bytes": ab,
                 "convention": "footprint: FP 8 taps x storage bytes per sample + 4 B per ray; "
                               "BP 4 taps x storage bytes per voxel-view + 4 B per voxel"}
+    bp_ab = algorithmic_bytes("bp", nvox, nviews, nrays, nsamp, s_vol, s_vol)
 
     extra = {"fp_ms": round(fp_ms, 3), "bp_ms": round(bp_ms, 3),
              "gups_fp": round(nvox * nviews / (fp_ms / 1e3) / 1e9, 2),
              "gups_bp": round(nvox * nviews / (bp_ms / 1e3) / 1e9, 2),
-             "fp_samples_per_s": round(nrays * nsamp / (fp_ms / 1e3), 1)}
-    if not args.no_variants and args.storage == "f32":
-        ms2, f2, b2, _, _ = measure("bf16", max(1, args.steps - 1), 1)
-        extra["bf16_storage"] = {"ms_per_step": round(ms2, 3), "fp_ms": round(f2, 3),
-                                 "bp_ms": round(b2, 3),
-                                 "gups": round(2.0 * nvox * nviews / (ms2 / 1e3) / 1e9, 2)}
+             "fp_samples_per_s": round(nrays * nsamp / (fp_ms / 1e3), 1),
+             "bp_roofline_frac": round(bp_ab / (bp_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if args.storage == "bf16":
+        # the plain FP32 path on the same inputs: its time, and the error of
+        # each BF16 stage against it (the FP32 path itself agrees with the
+        # FP64 oracle to ~1e-7 in the parity tests)
+        ms2, f2, b2, p32, v32 = measure("f32", max(1, args.steps - 1), 1)
+        g32 = 2.0 * nvox * nviews / (ms2 / 1e3) / 1e9
+        extra["plain_fp32"] = {"ms_per_step": round(ms2, 3), "fp_ms": round(f2, 3),
+                               "bp_ms": round(b2, 3), "gups": round(g32, 2),
+                               "speedup_of_bf16_storage": round(value / g32, 3)}
+        # FP stage: the step's projections; BP stage: BP of the FP32 sinogram
+        # with BF16 working copies vs without
+        ctx.storage("bf16")
+        vb = torch.empty_like(vol)
+        ctx.bp(g, p32, vb)
+        ctx.storage("f32")
+        torch.cuda.synchronize()
+        extra["stage_rel_l2_vs_fp32_path"] = {"fp": rel_err(proj, p32), "bp": rel_err(vb, v32),
+                                              "fp_then_bp": rel_err(v_out, v32)}
+        del p32, v32, vb
 
     if not args.no_variants:
         extra["fdk_prefilter"] = measure_filter(ctx, g, proj, nviews)
@@ -313,13 +345,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.storage == "f32" else "bf16-storage/f32-arith",
+        "dtype": "f32",
         "data": "synthetic (seeded ellipsoid phantom)",
         "config": {"workload": "%s: %dx%dx%d volume @ %.2f mm, %d views over 360 deg, "
                                "%dx%d detector @ %.2f mm, SAD %.0f, SDD %.0f; step = FP + BP"
                                % (cfg.name, g.nx, g.ny, g.nz, g.lx / g.nx, nviews, g.nu, g.nv,
                                   g.fov_u / g.nu, g.sad, g.sdd),
-                   "storage": args.storage, "samples_per_ray": nsamp},
+                   "storage": "%s working copies of volume and sinogram, FP32 arithmetic"
+                              % args.storage.upper(),
+                   "samples_per_ray": nsamp},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "latency": latency,

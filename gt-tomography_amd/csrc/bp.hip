@@ -62,7 +62,27 @@ __device__ __forceinline__ float bld<unsigned short>(__amdgpu_buffer_rsrc_t r, u
   return bf16_to_f32(__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0));
 }
 
-template <typename T, bool FBP, int KZ>
+// The four bilinear taps (u, v), (u+1, v), (u, v+1), (u+1, v+1) of a quad
+// texel (see k_pad_quad) from one load: 16 bytes for FP32, 8 for BF16.
+template <typename T>
+__device__ __forceinline__ float4 qld(__amdgpu_buffer_rsrc_t r, unsigned voff);
+template <>
+__device__ __forceinline__ float4 qld<float>(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0);
+  return make_float4(__uint_as_float((unsigned)v[0]), __uint_as_float((unsigned)v[1]),
+                     __uint_as_float((unsigned)v[2]), __uint_as_float((unsigned)v[3]));
+}
+template <>
+__device__ __forceinline__ float4 qld<unsigned short>(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0);
+  const unsigned w0 = (unsigned)v[0], w1 = (unsigned)v[1];
+  return make_float4(__uint_as_float(w0 << 16), __uint_as_float(w0 & 0xffff0000u),
+                     __uint_as_float(w1 << 16), __uint_as_float(w1 & 0xffff0000u));
+}
+
+// QUAD: projections in the quad layout (one load per voxel-view), else the
+// plain zero-bordered layout (four taps, two loads after merging).
+template <typename T, bool FBP, int KZ, bool QUAD>
 __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__restrict__ views,
                                                      const T *__restrict__ ppad,
                                                      float *__restrict__ vol, int accumulate) {
@@ -78,7 +98,7 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
   float acc[KZ];
 #pragma unroll
   for (int t = 0; t < KZ; ++t) acc[t] = 0.0f;
-  const unsigned esz = sizeof(T);
+  const unsigned esz = QUAD ? 4 * sizeof(T) : sizeof(T);  // bytes per (quad) texel
   const unsigned row_bytes = P.pnu * esz;  // < 2^24: 24-bit multiplies suffice
   const int view_bytes = (int)(P.pview * esz);
 
@@ -86,7 +106,7 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
   for (int k = 0; k < P.nviews; ++k) {
     const ViewF nxt = views[k + 1 < P.nviews ? k + 1 : k];  // prefetch the next view
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(ppad + (size_t)k * P.pview), (short)0, view_bytes, 0x00020000);
+        (void *)(ppad + (size_t)k * P.pview * (QUAD ? 4 : 1)), (short)0, view_bytes, 0x00020000);
     const float qx = fmaf(vw.c, px, vw.s * py);
     const float qy = fmaf(vw.c, py, -vw.s * px);
     const float mag = P.sdd * __builtin_amdgcn_rcpf(qy + P.sad);
@@ -106,8 +126,14 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
       const float tv = __builtin_amdgcn_fmed3f(fmaf((float)t, dtv, tv0), 0.0f, P.clv);
       const float av = __builtin_amdgcn_fractf(tv);
       const unsigned o = __umul24((unsigned)(int)tv, row_bytes) + ub;
-      const float a = bld<T>(rsrc, o, 0), b = bld<T>(rsrc, o + esz, 0);
-      const float c = bld<T>(rsrc, o, row_bytes), d = bld<T>(rsrc, o + esz, row_bytes);
+      float a, b, c, d;
+      if constexpr (QUAD) {
+        const float4 q = qld<T>(rsrc, o);
+        a = q.x; b = q.y; c = q.z; d = q.w;
+      } else {
+        a = bld<T>(rsrc, o, 0); b = bld<T>(rsrc, o + esz, 0);
+        c = bld<T>(rsrc, o, row_bytes); d = bld<T>(rsrc, o + esz, row_bytes);
+      }
       const float r0 = fmaf(au, b - a, a);
       const float r1 = fmaf(au, d - c, c);
       acc[t] = fmaf(w, fmaf(av, r1 - r0, r0), acc[t]);
@@ -158,6 +184,42 @@ __global__ void __launch_bounds__(256) k_pad_proj<unsigned short, unsigned short
     unsigned short val = 0;
     if (u >= 0 && u < nu && v >= 0 && v < nv) val = in[(size_t)u + (size_t)nu * ((size_t)v + (size_t)nv * k)];
     out[t] = val;
+  }
+}
+
+// Quad layout: texel (a, b) of the zero-bordered projection P (P(a, b) =
+// proj[a-1, b-1], 0 outside) holds P(a,b), P(a+1,b), P(a,b+1), P(a+1,b+1),
+// for a < nu+3, b < nv+3.  One thread per quad texel.
+template <typename TO, typename TI>
+__device__ __forceinline__ TO to_storage(TI v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ unsigned short to_storage<unsigned short, float>(float v) {
+  return f32_to_bf16(v);
+}
+
+// TI: the caller's projection type; T: the copy's (FP32 -> BF16 converts).
+template <typename TI, typename T>
+__global__ void __launch_bounds__(256) k_pad_quad(const TI *__restrict__ in, T *__restrict__ out,
+                                                  int nu, int nv, unsigned pnu, unsigned pnv,
+                                                  size_t total) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int a = (int)(t % pnu);
+    const int b = (int)((t / pnu) % pnv);
+    const size_t k = t / ((size_t)pnu * pnv);
+    const TI *pk = in + (size_t)nu * nv * k;
+    T q[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int u = a - 1 + (e & 1), v = b - 1 + (e >> 1);
+      q[e] = (u >= 0 && u < nu && v >= 0 && v < nv) ? to_storage<T>(pk[(size_t)u + (size_t)nu * v])
+                                                     : T(0);
+    }
+    T *o = out + 4 * t;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) o[e] = q[e];
   }
 }
 
@@ -227,14 +289,44 @@ BPParams make_params(const cbct_geom *g, int nviews) {
 
 constexpr int KZ = 8;
 
-template <typename T>
+template <typename T, bool QUAD>
 void launch_bp(const BPParams &P, dim3 grid, hipStream_t s, const ViewF *views, const T *pp,
                float *vol, int fbp, int acc) {
   dim3 block(BP_TX, BP_TY);
   if (fbp)
-    hipLaunchKernelGGL((k_bp<T, true, KZ>), grid, block, 0, s, P, views, pp, vol, acc);
+    hipLaunchKernelGGL((k_bp<T, true, KZ, QUAD>), grid, block, 0, s, P, views, pp, vol, acc);
   else
-    hipLaunchKernelGGL((k_bp<T, false, KZ>), grid, block, 0, s, P, views, pp, vol, acc);
+    hipLaunchKernelGGL((k_bp<T, false, KZ, QUAD>), grid, block, 0, s, P, views, pp, vol, acc);
+}
+
+// Padded copy in the selected layout (and storage type), then the kernel.
+// TI: the caller's projection type; T: the copy's.
+template <typename TI, typename T>
+int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int view0, int nviews,
+           const TI *proj, float *vol, int fbp, int accumulate, hipStream_t s) {
+  // the plain layout converts nothing: BF16 working copies use quads
+  const bool quad = ctx->bp_engine == 1 || sizeof(TI) != sizeof(T);
+  const unsigned pnu = (unsigned)padded_dim(g->nu), pnv = (unsigned)padded_dim(g->nv);
+  const size_t total = (size_t)P.pview * nviews;
+  int rc = cbct_reserve(ctx, ctx->proj_pad, total * sizeof(T) * (quad ? 4 : 1));
+  if (rc) return rc;
+  T *pp = static_cast<T *>(ctx->proj_pad.ptr);
+  if (quad) {
+    hipLaunchKernelGGL((k_pad_quad<TI, T>), stride_grid(total), dim3(256), 0, s, proj, pp, g->nu,
+                       g->nv, pnu, pnv, total);
+  } else {
+    hipLaunchKernelGGL((k_pad_proj<T, T>), stride_grid(total), dim3(256), 0, s,
+                       reinterpret_cast<const T *>(proj), pp, g->nu, g->nv, pnu, pnv, total);
+  }
+  CBCT_LAUNCH_CHECK(ctx, "k_pad_proj");
+  cbct_prof_mark(ctx, PROF_BP, 0, s);
+  if (quad)
+    launch_bp<T, true>(P, grid, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
+  else
+    launch_bp<T, false>(P, grid, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
+  cbct_prof_mark(ctx, PROF_BP, 1, s);
+  CBCT_LAUNCH_CHECK(ctx, "k_bp");
+  return CBCT_OK;
 }
 
 }  // namespace
@@ -266,30 +358,34 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
     if (!accumulate) return cbct_fill(ctx, (size_t)g->nx * g->ny * g->nz, vol, 0.0f, stream);
     return CBCT_OK;
   }
-  const size_t esz = dtype == CBCT_F32 ? 4 : 2;
-  const size_t total = pview * nviews;
-  rc = cbct_reserve(ctx, ctx->proj_pad, total * esz);
-  if (rc) return rc;
-  const unsigned pnu = (unsigned)padded_dim(g->nu), pnv = (unsigned)padded_dim(g->nv);
-  if (dtype == CBCT_F32) {
-    float *pp = static_cast<float *>(ctx->proj_pad.ptr);
-    hipLaunchKernelGGL((k_pad_proj<float, float>), stride_grid(total), dim3(256), 0, s,
-                       static_cast<const float *>(proj), pp, g->nu, g->nv, pnu, pnv, total);
-    CBCT_LAUNCH_CHECK(ctx, "k_pad_proj");
-    cbct_prof_mark(ctx, PROF_BP, 0, s);
-    launch_bp<float>(P, grid, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
-    cbct_prof_mark(ctx, PROF_BP, 1, s);
-  } else {
-    unsigned short *pp = static_cast<unsigned short *>(ctx->proj_pad.ptr);
-    hipLaunchKernelGGL((k_pad_proj<unsigned short, unsigned short>), stride_grid(total), dim3(256),
-                       0, s, static_cast<const unsigned short *>(proj), pp, g->nu, g->nv, pnu, pnv,
-                       total);
-    CBCT_LAUNCH_CHECK(ctx, "k_pad_proj");
-    cbct_prof_mark(ctx, PROF_BP, 0, s);
-    launch_bp<unsigned short>(P, grid, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
-    cbct_prof_mark(ctx, PROF_BP, 1, s);
-  }
-  CBCT_LAUNCH_CHECK(ctx, "k_bp");
+  // quad texels: one view of the padded copy must stay below 2^31 bytes
+  if (pview * 16 >= (1ull << 31))
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "Error: %s: detector too large (%d x %d)", who,
+                     g->nu, g->nv);
+  if (dtype == CBCT_BF16)
+    return run_bp<unsigned short, unsigned short>(ctx, g, P, grid, view0, nviews,
+                                                  static_cast<const unsigned short *>(proj), vol,
+                                                  fbp, accumulate, s);
+  if (ctx->work_dtype == CBCT_BF16)
+    return run_bp<float, unsigned short>(ctx, g, P, grid, view0, nviews,
+                                         static_cast<const float *>(proj), vol, fbp, accumulate, s);
+  return run_bp<float, float>(ctx, g, P, grid, view0, nviews, static_cast<const float *>(proj),
+                              vol, fbp, accumulate, s);
+}
+
+int cbct_storage(cbct_ctx ctx, int dtype) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (dtype != CBCT_F32 && dtype != CBCT_BF16)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_storage: unsupported storage type %d", dtype);
+  ctx->work_dtype = dtype;
+  return CBCT_OK;
+}
+
+int cbct_bp_engine(cbct_ctx ctx, int engine) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (engine != 0 && engine != 1)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_bp_engine: engine %d not available", engine);
+  ctx->bp_engine = engine;
   return CBCT_OK;
 }
 

@@ -102,6 +102,8 @@ def load():
         "cbct_bp": (I, [P, G, I, I, P, I, P, I, I, P]),
         "cbct_fp_transpose": (I, [P, G, I, I, P, P, F, I, P]),
         "cbct_fp_engine": (I, [P, I]),
+        "cbct_bp_engine": (I, [P, I]),
+        "cbct_storage": (I, [P, I]),
         "cbct_spacecarver": (I, [P, G, I, I, P, P, F, P]),
         "cbct_apply_mask": (I, [P, S, P, P, P]),
         "cbct_cosine_weights": (I, [G, P]),
@@ -235,8 +237,16 @@ class Context:
         return proj
 
     def fp_engine(self, engine):
-        """1 = detector-column sharing (default), 0 = one thread per ray."""
+        """0 = one thread per ray, >= 1 = detector-column sharing variants (1 default)."""
         self._check(self.lib.cbct_fp_engine(self.h, int(engine)))
+
+    def bp_engine(self, engine):
+        """1 = quad-texel projection copy (default), 0 = plain padded copy."""
+        self._check(self.lib.cbct_bp_engine(self.h, int(engine)))
+
+    def storage(self, dtype):
+        """Working-copy storage of the projectors for FP32 arrays: 'f32' or 'bf16'."""
+        self._check(self.lib.cbct_storage(self.h, CBCT_BF16 if dtype == "bf16" else CBCT_F32))
 
     def bp(self, g: Geom, proj, vol, view0=0, nviews=None, fbp=False, accumulate=False,
            stream=None):
@@ -354,6 +364,16 @@ class Context:
             a, y = args
             rc = fn(self.h, y.numel(), _ptr(a), _ptr(y), _stream(stream))
         self._check(rc)
+
+    def to_bf16(self, x, y, stream=None):
+        """y (bfloat16) = x (float32) rounded to nearest even."""
+        self._check(self.lib.cbct_to_bf16(self.h, x.numel(), _ptr(x), _ptr(y), _stream(stream)))
+        return y
+
+    def from_bf16(self, x, y, stream=None):
+        """y (float32) = x (bfloat16)."""
+        self._check(self.lib.cbct_from_bf16(self.h, x.numel(), _ptr(x), _ptr(y), _stream(stream)))
+        return y
 
     def reduce(self, op, a, b=None, stream=None):
         out = ctypes.c_double()

@@ -60,7 +60,7 @@ double ms(Clock::time_point a, Clock::time_point b) {
 size_t footprint_bytes(const gtt_request &r) {
   const size_t nvox = (size_t)r.geom.nx * r.geom.ny * r.geom.nz;
   const size_t nproj = (size_t)(r.geom.nu + 3) * (r.geom.nv + 3) * (size_t)r.nviews;
-  return 4 * (6 * nvox + 5 * nproj) + (64u << 20);
+  return 4 * (6 * nvox + 8 * nproj) + (64u << 20);
 }
 
 void run_request(Job &j) {

@@ -97,10 +97,28 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
             int proj_dtype, float *vol, int fbp, int accumulate, void *stream);
 
 /* Forward-projector engine: 1 = detector-column sharing (default; one wave per
- * detector column interpolates the shared z-line once per step), 0 = one
- * thread per ray.  Both give bitwise-identical projections.  No reference
- * counterpart (the reference has one texture-based kernel). */
+ * detector column interpolates the shared z-line once per step, in the
+ * variant measured fastest for the volume's storage type), 2 = the other
+ * column-sharing variant, 0 = one thread per ray.  All give bitwise-identical
+ * projections.  No reference counterpart (the reference has one
+ * texture-based kernel). */
 int cbct_fp_engine(cbct_ctx ctx, int engine);
+
+/* Backprojector engine: 1 = quad-texel copy of the projections (default; the
+ * four bilinear taps of a voxel-view in one 16-byte (FP32) or 8-byte (BF16)
+ * load, 4x the padded projection workspace), 0 = plain zero-bordered copy.
+ * Both give bitwise-identical volumes.  No reference counterpart (the
+ * reference samples a layered texture). */
+int cbct_bp_engine(cbct_ctx ctx, int engine);
+
+/* Storage type of the projectors' working copies (the zero-bordered volume
+ * copy the forward projector gathers from, the projection copy the
+ * backprojector gathers from) when the caller's arrays are FP32:
+ * CBCT_F32 (default) or CBCT_BF16 (the copy kernels round to nearest even;
+ * arithmetic stays FP32; relative error ~1e-3 per projection).  BF16 arrays
+ * passed by the caller are always used as they are.  No reference
+ * counterpart (the reference samples FP32 textures). */
+int cbct_storage(cbct_ctx ctx, int dtype);
 
 /* Exact transpose of cbct_fp (ray-driven scatter, fp32 atomics):
  * vol = (accumulate ? vol : 0) + A^T proj.  No reference counterpart: the

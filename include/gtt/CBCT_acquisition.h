@@ -8,6 +8,7 @@
 #include <ostream>
 #include <set>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "gtt/ndarray.h"
@@ -56,10 +57,19 @@ class CBCT_acquisition {
   std::shared_ptr<CBCT_geometry> get_geometry() { return geometry_; }
   void set_projections(std::shared_ptr<hoCuNDArray<float>> p) { projections_ = std::move(p); }
   std::shared_ptr<hoCuNDArray<float>> get_projections() { return projections_; }
+  // Air scan (I0) if the file holds one (nullptr otherwise).
+  std::shared_ptr<hoCuNDArray<float>> get_I0() { return I0_; }
+  void set_I0(std::shared_ptr<hoCuNDArray<float>> I0) { I0_ = std::move(I0); }
+
+  // HDF5 acquisition files (geometry format 2, projection format 1):
+  // xray/CBCT_acquisition.h:177-333.  Defined in libgtt_h5.so.
+  void load(const std::string &filename);
+  void save(const std::string &filename);
 
  private:
   std::shared_ptr<CBCT_geometry> geometry_;
   std::shared_ptr<hoCuNDArray<float>> projections_;
+  std::shared_ptr<hoCuNDArray<float>> I0_;
 };
 
 class CBCT_binning {
@@ -100,6 +110,11 @@ class CBCT_binning {
     for (unsigned int i = 0; i < nproj; ++i) b[i] = i;
     binning_.push_back(b);
   }
+  // HDF5 binning files (format 1: /binning_dataformat_version, /numBins,
+  // /bin_1.. /bin_N): xray/CBCT_binning.h:106-157; save writes the layout of
+  // python/l1shroud.py:139-150.  Defined in libgtt_h5.so.
+  void load(const std::string &filename);
+  void save(const std::string &filename) const;
   std::shared_ptr<CBCT_binning> get_3d_binning() const {
     std::set<unsigned int> u;
     for (auto &b : binning_) u.insert(b.begin(), b.end());

@@ -88,7 +88,7 @@ def test_request_footprint_and_server_without_gpu():
     req = S.Request(kind=S.REQ_OS_SART, geom=g, angles=spec.angles(),
                     projections=np.zeros(g.proj_shape(30), np.float32))
     nvox, nproj = 40 ** 3, (64 + 3) * (48 + 3) * 30
-    assert req.footprint() == 4 * (6 * nvox + 5 * nproj) + (64 << 20)
+    assert req.footprint() == 4 * (6 * nvox + 8 * nproj) + (64 << 20)
     try:
         import torch
         if torch.cuda.is_available():

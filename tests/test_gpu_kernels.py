@@ -93,14 +93,15 @@ def test_fp_engines_bitwise_identical(ctx, name, dtype):
     tdt = torch.float32 if dtype == "f32" else torch.bfloat16
     ctx.set_views(angles, offsets)
     outs = []
-    for e in (0, 1):
+    for e in (0, 1, 2):   # per ray; the column-sharing variants
         ctx.fp_engine(e)
         out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
         ctx.fp(g, dev(vol, tdt), out)
         torch.cuda.synchronize()
         outs.append(out)
     ctx.fp_engine(1)
-    assert torch.equal(outs[0], outs[1])
+    for e in (1, 2):
+        assert torch.equal(outs[0], outs[e]), e
     if dtype == "f32":
         assert rel_l2(host(outs[1]), O.fp(to_oracle(g), angles, offsets, vol)) <= TOL
 
@@ -118,6 +119,63 @@ def test_bp_parity(ctx, name, fbp):
     err = rel_l2(host(out), ref)
     print("bp rel-L2", name, fbp, err)
     assert err <= TOL
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_bp_engines_bitwise_identical(ctx, name, dtype):
+    """Quad-texel and plain padded projection copies give identical bits
+    (FBP and plain, FP32 and BF16 storage, accumulating into a volume)."""
+    g, angles, offsets = make_case(name)
+    print_case("bp engines", name, dtype, g)
+    proj = np.random.default_rng(4).random(g.proj_shape(len(angles)))
+    tdt = torch.float32 if dtype == "f32" else torch.bfloat16
+    ctx.set_views(angles, offsets)
+    start = torch.from_numpy(np.random.default_rng(6).random(g.vol_shape).astype(np.float32)).cuda()
+    for fbp in (False, True):
+        outs = []
+        for e in (0, 1):
+            ctx.bp_engine(e)
+            out = start.clone()
+            ctx.bp(g, dev(proj, tdt), out, fbp=fbp, accumulate=True)
+            torch.cuda.synchronize()
+            outs.append(out)
+        ctx.bp_engine(1)
+        assert torch.equal(outs[0], outs[1]), fbp
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_bf16_working_copies(ctx, name):
+    """FP32 arrays with BF16 working copies (cbct_storage) give the same bits
+    as BF16 arrays rounded by the caller, for FP and (F)BP, and stay within the
+    BF16 bound of the FP32 result."""
+    g, angles, offsets = make_case(name)
+    print_case("bf16 working copies", name, g)
+    vol = phantom_volume(g) + 0.01
+    proj = np.random.default_rng(12).random(g.proj_shape(len(angles)))
+    ctx.set_views(angles, offsets)
+    outs = {}
+    for mode in ("explicit", "storage", "f32"):
+        ctx.storage("bf16" if mode == "storage" else "f32")
+        v = dev(vol, torch.bfloat16) if mode == "explicit" else dev(vol)
+        p = dev(proj, torch.bfloat16) if mode == "explicit" else dev(proj)
+        fp = torch.zeros(g.proj_shape(len(angles)), device="cuda")
+        ctx.fp(g, v, fp)
+        bps = []
+        for fbp in (False, True):
+            b = torch.empty(g.vol_shape, device="cuda")
+            ctx.bp(g, p, b, fbp=fbp)
+            bps.append(b)
+        torch.cuda.synchronize()
+        outs[mode] = (fp, bps)
+    ctx.storage("f32")
+    assert torch.equal(outs["explicit"][0], outs["storage"][0])
+    for a, b in zip(outs["explicit"][1], outs["storage"][1]):
+        assert torch.equal(a, b)
+    e_fp = rel_l2(host(outs["storage"][0]), host(outs["f32"][0]))
+    e_bp = rel_l2(host(outs["storage"][1][0]), host(outs["f32"][1][0]))
+    print("bf16 working copies rel-L2 vs f32: fp", e_fp, "bp", e_bp)
+    assert e_fp <= 3e-3 and e_bp <= 3e-3
 
 
 def test_fp_bp_accumulate_and_view_ranges(ctx):
