@@ -85,10 +85,16 @@ __device__ __forceinline__ float4 qld<unsigned short>(__amdgpu_buffer_rsrc_t r, 
 template <typename T, bool FBP, int KZ, bool QUAD>
 __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__restrict__ views,
                                                      const T *__restrict__ ppad,
-                                                     float *__restrict__ vol, int accumulate) {
-  const int x = blockIdx.x * BP_TX + threadIdx.x;
-  const int y = blockIdx.y * BP_TY + threadIdx.y;
-  const int z0 = blockIdx.z * KZ;
+                                                     float *__restrict__ vol, int accumulate,
+                                                     unsigned gx, unsigned gy) {
+  // linear tile id, x fastest, z slowest.  Consecutive tiles land on
+  // different XCDs, so all XCDs work on one band of the volume -- and one
+  // band of detector rows per view -- at a time (measured faster than giving
+  // each XCD its own z-slab).
+  const unsigned L = blockIdx.x;
+  const int x = (int)(L % gx) * BP_TX + threadIdx.x;
+  const int y = (int)((L / gx) % gy) * BP_TY + threadIdx.y;
+  const int z0 = (int)(L / (gx * gy)) * KZ;
   if (x >= P.nx || y >= P.ny) return;
   // voxel centre in mm (xray/conebeam_projection.cu:945-959)
   const float px = ((x + 0.5f) / P.nx - 0.5f) * P.lx;
@@ -293,10 +299,13 @@ template <typename T, bool QUAD>
 void launch_bp(const BPParams &P, dim3 grid, hipStream_t s, const ViewF *views, const T *pp,
                float *vol, int fbp, int acc) {
   dim3 block(BP_TX, BP_TY);
+  const unsigned nblocks = grid.x * grid.y * grid.z;
   if (fbp)
-    hipLaunchKernelGGL((k_bp<T, true, KZ, QUAD>), grid, block, 0, s, P, views, pp, vol, acc);
+    hipLaunchKernelGGL((k_bp<T, true, KZ, QUAD>), dim3(nblocks), block, 0, s, P, views, pp, vol, acc,
+                       grid.x, grid.y);
   else
-    hipLaunchKernelGGL((k_bp<T, false, KZ, QUAD>), grid, block, 0, s, P, views, pp, vol, acc);
+    hipLaunchKernelGGL((k_bp<T, false, KZ, QUAD>), dim3(nblocks), block, 0, s, P, views, pp, vol,
+                       acc, grid.x, grid.y);
 }
 
 // Padded copy in the selected layout (and storage type), then the kernel.

@@ -108,6 +108,17 @@ __device__ __forceinline__ unsigned short f32_to_bf16(float f) {
 // clamped texel coordinate in [-1, n] never needs a bounds test.
 static inline size_t padded_dim(int n) { return static_cast<size_t>(n) + 3; }
 
+// XCD-aware block order.  The hardware deals linear block ids round-robin to
+// the 8 XCDs (block b runs on XCD b mod 8).  This maps b to a logical id such
+// that XCD x processes the contiguous logical range [start_x, start_x +
+// count_x): neighbouring logical blocks -- which read neighbouring data --
+// then share one XCD's L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
+  const unsigned q = nblocks >> 3, rem = nblocks & 7u;
+  const unsigned x = b & 7u, idx = b >> 3;
+  return x * q + (x < rem ? x : rem) + idx;
+}
+
 // Launch helpers
 static inline unsigned int ceil_div(size_t a, size_t b) {
   return static_cast<unsigned int>((a + b - 1) / b);

@@ -168,15 +168,6 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 constexpr int MB = 128, NB = 128, KB = 32;
 constexpr int LDK = KB + 8;  // LDS row pitch in halves (80 bytes: 16-byte aligned, staggered banks)
 
-// Map the linear block id so that XCD x (which receives blocks b == x mod 8)
-// processes the contiguous logical range [start_x, start_x + count_x): the
-// NB-column tiles of one row tile then share that XCD's L2 copy of the rows.
-__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
-  const unsigned q = nblocks >> 3, rem = nblocks & 7u;
-  const unsigned x = b & 7u, idx = b >> 3;
-  return x * q + (x < rem ? x : rem) + idx;
-}
-
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));

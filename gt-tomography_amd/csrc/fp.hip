@@ -187,14 +187,6 @@ __global__ void __launch_bounds__(256) k_fp_ray_transpose(FPParams P, const View
 }
 
 // ------------------------------------------------------------------ engine 1
-// Map the linear block id so that XCD x (which receives blocks b == x mod 8)
-// processes the contiguous logical range [start_x, start_x + count_x).
-__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
-  const unsigned q = nblocks >> 3, rem = nblocks & 7u;
-  const unsigned x = b & 7u, idx = b >> 3;
-  return x * q + (x < rem ? x : rem) + idx;
-}
-
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
