@@ -141,12 +141,24 @@ def serving_leg(ctx, lanes=4, n_batch=16, n_stream=24, load=0.8):
     rp, wp = run(reqs, 1)
     vp = [dict(volume=r["volume"].copy()) for r in rp]
     rs, ws = run(reqs, lanes)
-    out["batch"] = {"plain": S.summarize(reqs, rp, wp), "served": S.summarize(reqs, rs, ws),
+    # served with BF16 working copies, against the plain FP32 results
+    for r in reqs:
+        r.storage = "bf16"
+    rb, wb = run(reqs, lanes)
+    for r in reqs:
+        r.storage = "f32"
+    errs = [float(np.linalg.norm(b["volume"] - p["volume"]) / np.linalg.norm(p["volume"]))
+            for b, p in zip(rb, vp)]
+    sb = S.summarize(reqs, rb, wb)
+    sb["rel_l2_vs_plain_fp32"] = {"max": max(errs), "mean": float(np.mean(errs))}
+    sp = S.summarize(reqs, rp, wp)
+    out["batch"] = {"plain": sp, "served": S.summarize(reqs, rs, ws), "served_bf16": sb,
+                    "served_bf16_gups_vs_plain_fp32": round(sb["gups"] / sp["gups"], 3),
                     "identical": same(vp, rs),
                     "sizes": "%d OS-SART scans, edge %d..%d, %d..%d views" % (
                         len(reqs), min(r.geom.nx for r in reqs), max(r.geom.nx for r in reqs),
                         min(len(r.angles) for r in reqs), max(len(r.angles) for r in reqs))}
-    del reqs, rp, rs, vp
+    del reqs, rp, rs, rb, vp
     t0 = time.perf_counter()
     reqs = [S.make_request(s, ctx) for s in S.uneven_scans(n_stream, seed=1004, p_fdk=0.6)]
     gen_s += time.perf_counter() - t0

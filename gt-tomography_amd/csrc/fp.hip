@@ -640,13 +640,14 @@ void launch_col_w(const FPParams &P, int variant, hipStream_t s, const ViewD *vi
     launch_col<T, RPL, false, 1>(P, s, views, vt, proj, acc, nviews, vt_bytes);
 }
 
-// Rays per lane: whole 64-row blocks, at most 8 per lane (rows are split into
-// groups beyond that to keep registers, and so occupancy, in check).
+// Rays per lane: whole 64-row blocks, at most `max_rpl` per lane (1..12;
+// rows are split into groups beyond that).  More rows per wave share one
+// z-line load; fewer keep registers, and so occupancy, in check.
 template <typename T>
-void dispatch_col(const FPParams &P, int variant, hipStream_t s, const ViewD *views, const T *vt,
-                  float *proj, int acc, int nviews, unsigned vt_bytes) {
+void dispatch_col(const FPParams &P, int variant, int max_rpl, hipStream_t s, const ViewD *views,
+                  const T *vt, float *proj, int acc, int nviews, unsigned vt_bytes) {
   const int need = (P.nv + 63) / 64;
-  const int groups = (need + 7) / 8;
+  const int groups = (need + max_rpl - 1) / max_rpl;
   const int rpl = (need + groups - 1) / groups;
   switch (rpl) {
     case 1: launch_col_w<T, 1>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
@@ -656,7 +657,11 @@ void dispatch_col(const FPParams &P, int variant, hipStream_t s, const ViewD *vi
     case 5: launch_col_w<T, 5>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
     case 6: launch_col_w<T, 6>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
     case 7: launch_col_w<T, 7>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
-    default: launch_col_w<T, 8>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 8: launch_col_w<T, 8>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 9: launch_col_w<T, 9>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 10: launch_col_w<T, 10>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 11: launch_col_w<T, 11>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    default: launch_col_w<T, 12>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
   }
 }
 
@@ -703,7 +708,8 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
                      P.px, P.pz);
   CBCT_LAUNCH_CHECK(ctx, "k_pad_transpose");
   cbct_prof_mark(ctx, PROF_FP, 0, s);
-  dispatch_col<T>(P, variant, s, views, vt, proj, accumulate, nviews, (unsigned)vt_bytes);
+  dispatch_col<T>(P, variant, ctx->fp_max_rpl, s, views, vt, proj, accumulate, nviews,
+                  (unsigned)vt_bytes);
   cbct_prof_mark(ctx, PROF_FP, 1, s);
   CBCT_LAUNCH_CHECK(ctx, "k_fp_col");
   return CBCT_OK;
@@ -718,6 +724,14 @@ int cbct_fp_engine(cbct_ctx ctx, int engine) {
   if (engine < 0 || engine > 2)
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_fp_engine: engine %d not available", engine);
   ctx->fp_engine = engine;
+  return CBCT_OK;
+}
+
+int cbct_fp_tile(cbct_ctx ctx, int max_rows_per_lane) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (max_rows_per_lane < 1 || max_rows_per_lane > 12)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_fp_tile: rows per lane %d outside 1..12", max_rows_per_lane);
+  ctx->fp_max_rpl = max_rows_per_lane;
   return CBCT_OK;
 }
 

@@ -102,6 +102,7 @@ def load():
         "cbct_bp": (I, [P, G, I, I, P, I, P, I, I, P]),
         "cbct_fp_transpose": (I, [P, G, I, I, P, P, F, I, P]),
         "cbct_fp_engine": (I, [P, I]),
+        "cbct_fp_tile": (I, [P, I]),
         "cbct_bp_engine": (I, [P, I]),
         "cbct_storage": (I, [P, I]),
         "cbct_spacecarver": (I, [P, G, I, I, P, P, F, P]),
@@ -239,6 +240,10 @@ class Context:
     def fp_engine(self, engine):
         """0 = one thread per ray, >= 1 = detector-column sharing variants (1 default)."""
         self._check(self.lib.cbct_fp_engine(self.h, int(engine)))
+
+    def fp_tile(self, max_rows_per_lane):
+        """Column engine: most detector rows per lane (1..12)."""
+        self._check(self.lib.cbct_fp_tile(self.h, int(max_rows_per_lane)))
 
     def bp_engine(self, engine):
         """1 = quad-texel projection copy (default), 0 = plain padded copy."""

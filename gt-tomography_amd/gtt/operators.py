@@ -32,6 +32,8 @@ def load():
     D = ctypes.POINTER(ctypes.c_double)
     L.gtt_last_error.restype = ctypes.c_char_p
     L.gtt_last_error.argtypes = []
+    L.gtt_set_default_storage.argtypes = [I]
+    L.gtt_set_default_storage.restype = I
     L.gtt_fdk_reconstruct_3d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, D]
     L.gtt_os_sart.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, P, D]
     L.gtt_os_mom.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, F, F, I, I, P]
@@ -47,6 +49,12 @@ def load():
         getattr(L, n).restype = I
     _lib = L
     return L
+
+
+def set_default_storage(dtype):
+    """Working copies of the projectors ('f32' or 'bf16') for reconstructions
+    this thread starts from now on."""
+    _check(load().gtt_set_default_storage(capi.CBCT_BF16 if dtype == "bf16" else capi.CBCT_F32))
 
 
 def _check(rc):

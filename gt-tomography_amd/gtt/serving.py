@@ -33,7 +33,8 @@ class _Request(ctypes.Structure):
                 ("projections", ctypes.c_void_p), ("filter", ctypes.c_void_p),
                 ("samples_per_pixel", ctypes.c_float), ("subsets", ctypes.c_int),
                 ("iterations", ctypes.c_int), ("beta", ctypes.c_float), ("gamma", ctypes.c_float),
-                ("nonneg", ctypes.c_int), ("volume_out", ctypes.c_void_p)]
+                ("nonneg", ctypes.c_int), ("volume_out", ctypes.c_void_p),
+                ("storage", ctypes.c_int)]
 
 
 class _Result(ctypes.Structure):
@@ -79,6 +80,7 @@ class Request:
     beta: float = 1.0
     gamma: float = 0.0
     nonneg: bool = True
+    storage: str = "f32"                 # projectors' working copies: "f32" or "bf16"
     volume: np.ndarray | None = None     # filled on completion
 
     def updates(self):
@@ -102,7 +104,8 @@ class Request:
         p = lambda a: None if a is None else a.ctypes.data
         return _Request(self.kind, g, len(self.angles), p(self.angles), p(self.offsets),
                         p(self.projections), p(self.filter), self.samples_per_pixel, self.subsets,
-                        self.iterations, self.beta, self.gamma, int(self.nonneg), p(self.volume))
+                        self.iterations, self.beta, self.gamma, int(self.nonneg), p(self.volume),
+                        1 if self.storage == "bf16" else 0)
 
     def footprint(self):
         return int(_lib().gtt_request_footprint(ctypes.byref(self._struct())))

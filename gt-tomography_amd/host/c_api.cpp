@@ -48,6 +48,15 @@ extern "C" {
 
 const char *gtt_last_error(void) { return g_err.c_str(); }
 
+int gtt_set_default_storage(int dtype) {
+  if (dtype != CBCT_F32 && dtype != CBCT_BF16) {
+    g_err = "gtt_set_default_storage: storage must be CBCT_F32 or CBCT_BF16";
+    return -1;
+  }
+  set_default_storage(dtype);
+  return 0;
+}
+
 int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
                            const unsigned *bin_index, const int *bin_sizes, int nbins,
                            const float *projections, const float *filter, float spp,

@@ -50,6 +50,7 @@ void upload_views(cbct_ctx ctx, const std::vector<float> &a, const std::vector<f
 // ---------------------------------------------------------------------------
 cuConebeamProjectionOperator::cuConebeamProjectionOperator() {
   check(cbct_ctx_create(current_device(), &ctx_), nullptr);
+  check(cbct_storage(ctx_, default_storage()), ctx_);
 }
 
 cuConebeamProjectionOperator::~cuConebeamProjectionOperator() { cbct_ctx_destroy(ctx_); }

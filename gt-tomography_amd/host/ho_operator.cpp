@@ -21,6 +21,7 @@ const float kPi = 3.14159265358979f;
 
 hoCuConebeamProjectionOperator::hoCuConebeamProjectionOperator() {
   check(cbct_ctx_create(current_device(), &ctx_), nullptr);
+  check(cbct_storage(ctx_, default_storage()), ctx_);
 }
 
 hoCuConebeamProjectionOperator::~hoCuConebeamProjectionOperator() { cbct_ctx_destroy(ctx_); }

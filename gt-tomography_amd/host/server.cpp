@@ -63,8 +63,17 @@ size_t footprint_bytes(const gtt_request &r) {
   return 4 * (6 * nvox + 8 * nproj) + (64u << 20);
 }
 
+// The request's storage type is the lane's default while its operators are
+// built (they keep it); restored afterwards.
+struct StorageScope {
+  int saved;
+  explicit StorageScope(int dtype) : saved(default_storage()) { set_default_storage(dtype); }
+  ~StorageScope() { set_default_storage(saved); }
+};
+
 void run_request(Job &j) {
   const gtt_request &r = j.req;
+  StorageScope storage(r.storage == CBCT_BF16 ? CBCT_BF16 : CBCT_F32);
   Scan s = make_scan(&r.geom, r.angles_deg, r.offsets, r.nviews, nullptr, nullptr, 0, r.projections);
   if (r.kind == GTT_REQ_FDK) {
     std::shared_ptr<std::vector<float>> F;
@@ -208,6 +217,10 @@ int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id) {
   }
   if (req->kind != GTT_REQ_FDK && req->kind != GTT_REQ_OS_SART) {
     s->err = "gtt_server_submit: unknown request kind";
+    return -1;
+  }
+  if (req->storage != CBCT_F32 && req->storage != CBCT_BF16) {
+    s->err = "gtt_server_submit: storage must be CBCT_F32 or CBCT_BF16";
     return -1;
   }
   if (s->stop) {

@@ -104,6 +104,12 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
  * texture-based kernel). */
 int cbct_fp_engine(cbct_ctx ctx, int engine);
 
+/* Column-engine tile: the most detector rows one lane carries (1..12,
+ * default 8).  Columns with more rows than 64 x this are split into row
+ * groups, each with its own z-line loads.  Results do not depend on it.  No
+ * reference counterpart. */
+int cbct_fp_tile(cbct_ctx ctx, int max_rows_per_lane);
+
 /* Backprojector engine: 1 = quad-texel copy of the projections (default; the
  * four bilinear taps of a voxel-view in one 16-byte (FP32) or 8-byte (BF16)
  * load, 4x the padded projection workspace), 0 = plain zero-bordered copy.

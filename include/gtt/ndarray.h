@@ -28,6 +28,10 @@ void *current_stream();
 void set_current_stream(void *stream);
 // Destroy the calling thread's contexts (worker threads, before they exit).
 void release_thread_resources();
+// Storage type (CBCT_F32 or CBCT_BF16) of the projectors' working copies for
+// projection operators the calling thread creates from now on (default FP32).
+void set_default_storage(int dtype);
+int default_storage();
 // HIP device current on the calling thread.
 int current_device();
 // Throw std::runtime_error with the library's message when rc != 0.

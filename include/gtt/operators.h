@@ -136,6 +136,9 @@ class cuConebeamProjectionOperator : public linearOperator<cuNDArray<float>> {
   std::shared_ptr<cuNDArray<unsigned char>> calculate_mask(cuNDArray<float> *projections, float limit);
   bool is_short_scan() const { return short_scan_; }
   cbct_ctx context() const { return ctx_; }
+  // Working-copy storage of the projectors (CBCT_F32 / CBCT_BF16); new
+  // operators take the creating thread's default_storage().
+  void set_storage(int dtype) { check(cbct_storage(ctx_, dtype), ctx_); }
 
  protected:
   cbct_geom geom_for(size_t nx, size_t ny, size_t nz, size_t nu, size_t nv) const;
@@ -195,6 +198,8 @@ class hoCuConebeamProjectionOperator : public linearOperator<hoCuNDArray<float>>
   void set_binning(std::shared_ptr<CBCT_binning> b);
   std::shared_ptr<CBCT_binning> get_binning() { return binning_; }
   bool is_short_scan() const { return short_scan_; }
+  // Working-copy storage of the projectors (CBCT_F32 / CBCT_BF16)
+  void set_storage(int dtype) { check(cbct_storage(ctx_, dtype), ctx_); }
 
  protected:
   void build_view_table();
@@ -244,6 +249,9 @@ class CBSubsetOperator : public subsetOperator<cuNDArray<float>> {
   }
   void set_mask(std::shared_ptr<cuNDArray<unsigned char>> mask) {
     for (auto &op : operators_) op->set_mask(mask);
+  }
+  void set_storage(int dtype) {
+    for (auto &op : operators_) op->set_storage(dtype);
   }
   const std::vector<unsigned int> &get_permutation() const { return permutations_; }
   std::shared_ptr<cuConebeamProjectionOperator> get_subset_operator(int s) { return operators_[s]; }

@@ -28,6 +28,11 @@ extern "C" {
 
 const char *gtt_last_error(void);
 
+/* Storage type (CBCT_F32 or CBCT_BF16) of the projectors' working copies in
+ * every reconstruction the calling thread starts from now on (see
+ * cbct_storage).  No reference counterpart. */
+int gtt_set_default_storage(int dtype);
+
 int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles_deg, const float *offsets,
                            int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
                            const float *projections, const float *filter /* nu+1 or NULL */,
@@ -88,6 +93,7 @@ typedef struct {
   float beta, gamma;
   int nonneg;
   float *volume_out;          /* host, nx x ny x nz */
+  int storage;                /* CBCT_F32 (0) or CBCT_BF16 (1) working copies */
 } gtt_request;
 
 typedef struct {

@@ -9,6 +9,8 @@
   refused at submit; with a budget of one request, requests run one at a
   time in arrival order.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -107,3 +109,37 @@ def test_failed_request_reports_and_server_continues(small_requests):
         with pytest.raises(RuntimeError, match="subsets"):
             srv.wait(ib)
         assert np.all(np.isfinite(srv.wait(ig)["volume"]))
+
+
+def test_served_bf16_storage_equals_alone(small_requests):
+    """A request with BF16 working copies, served beside FP32 requests,
+    equals the same reconstruction run alone with BF16 working copies
+    (bitwise), and stays within the BF16 bound of the FP32 result."""
+    reqs = [r for r in small_requests if r.kind == S.REQ_OS_SART][:2] + \
+           [r for r in small_requests if r.kind == S.REQ_FDK][:1]
+    print_case("server bf16 storage", [(r.kind, r.geom.nx) for r in reqs])
+    ref32 = [alone(r) for r in reqs]
+    ops.set_default_storage("bf16")
+    try:
+        ref16 = [alone(r) for r in reqs]
+    finally:
+        ops.set_default_storage("f32")
+    mixed = []
+    for r in reqs:
+        r.volume = None
+        r.storage = "bf16"
+        mixed.append(r)
+    # an FP32 request served beside them (its own copy: the others are BF16)
+    extra = dataclasses.replace(small_requests[0], volume=None, storage="f32")
+    try:
+        with S.Server(lanes=3) as srv:
+            res, _ = S.run_stream(srv, mixed + [extra])
+    finally:
+        for r in reqs:
+            r.storage = "f32"
+    for r, out, w16, w32 in zip(mixed, res, ref16, ref32):
+        assert np.array_equal(out["volume"], w16), r.kind
+        err = rel_l2(out["volume"], w32)
+        print("bf16 working copies vs fp32, kind", r.kind, "rel-L2", err)
+        assert err <= 2e-2
+    assert np.array_equal(res[-1]["volume"], alone(extra))
