@@ -35,6 +35,7 @@ def load():
     L.gtt_set_default_storage.argtypes = [I]
     L.gtt_set_default_storage.restype = I
     L.gtt_fdk_reconstruct_3d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, D]
+    L.gtt_fdk_reconstruct_4d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, P, D]
     L.gtt_os_sart.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, P, D]
     L.gtt_os_mom.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, F, F, I, I, P]
     L.gtt_subset_op_create.argtypes = [I, ctypes.POINTER(P)]
@@ -43,7 +44,7 @@ def load():
     L.gtt_subset_op_views.argtypes = [P, I]
     L.gtt_subset_op_mult_M.argtypes = [P, P, P, I, I]
     L.gtt_subset_op_mult_MH.argtypes = [P, P, P, I, I]
-    for n in ("gtt_fdk_reconstruct_3d", "gtt_os_sart", "gtt_os_mom", "gtt_subset_op_create",
+    for n in ("gtt_fdk_reconstruct_3d", "gtt_fdk_reconstruct_4d", "gtt_os_sart", "gtt_os_mom", "gtt_subset_op_create",
               "gtt_subset_op_destroy", "gtt_subset_op_setup", "gtt_subset_op_views",
               "gtt_subset_op_mult_M", "gtt_subset_op_mult_MH"):
         getattr(L, n).restype = I
@@ -98,6 +99,30 @@ def fdk_reconstruct_3d(g: capi.Geom, angles, offsets, projections, filt=None, bi
     _check(L.gtt_fdk_reconstruct_3d(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb,
                                     _p(p), _p(F), samples_per_pixel, _p(out), ctypes.byref(scale)))
     return out, scale.value
+
+
+def fdk_reconstruct_4d(g: capi.Geom, angles, offsets, projections, bins, filt=None,
+                       samples_per_pixel=1.5):
+    """FDK_reconstruct_4d (McKinnon-Bates): returns (volumes (nbins, nz, ny, nx),
+    the FBP over the union of the bins (nz, ny, nx), least-squares scale)."""
+    L = load()
+    a, o = _views(angles, offsets)
+    if not bins:
+        raise ValueError("fdk_reconstruct_4d needs at least one bin")
+    idx, sizes, nb = _bins(bins)
+    p = np.ascontiguousarray(projections, np.float32)
+    if p.shape != (a.size, g.nv, g.nu):
+        raise ValueError(f"projections must be (nviews, nv, nu) = {(a.size, g.nv, g.nu)}, got {p.shape}")
+    F = None if filt is None else np.ascontiguousarray(filt, np.float32)
+    if F is not None and F.size != g.nu + 1:
+        raise ValueError("the frequency filter holds nu+1 taps")
+    out = np.zeros((nb, g.nz, g.ny, g.nx), np.float32)
+    f3 = np.zeros((g.nz, g.ny, g.nx), np.float32)
+    scale = ctypes.c_double()
+    _check(L.gtt_fdk_reconstruct_4d(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb,
+                                    _p(p), _p(F), samples_per_pixel, _p(out), _p(f3),
+                                    ctypes.byref(scale)))
+    return out, f3, scale.value
 
 
 def os_sart(g: capi.Geom, angles, offsets, projections, nsubsets, iterations, beta=1.0, gamma=0.0,

@@ -72,6 +72,23 @@ int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles, const float 
   });
 }
 
+int gtt_fdk_reconstruct_4d(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
+                           const unsigned *bin_index, const int *bin_sizes, int nbins,
+                           const float *projections, const float *filter, float spp,
+                           float *volume_out, float *fdk3d_out, double *scale_out) {
+  return guarded([&] {
+    if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    std::shared_ptr<std::vector<float>> F;
+    if (filter) F = std::make_shared<std::vector<float>>(filter, filter + g->nu + 1);
+    const std::vector<size_t> dims3(s.vol_dims.begin(), s.vol_dims.begin() + 3);
+    FDK4DResult r = FDK_reconstruct_4d(s.acq, s.binning, dims3, s.dims_mm, F, spp > 0 ? spp : 1.5f);
+    std::memcpy(volume_out, r.image->get_data_ptr(), r.image->get_number_of_bytes());
+    if (fdk3d_out) std::memcpy(fdk3d_out, r.fdk_3d->get_data_ptr(), r.fdk_3d->get_number_of_bytes());
+    if (scale_out) *scale_out = r.scale;
+  });
+}
+
 int gtt_os_sart(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
                 const unsigned *bin_index, const int *bin_sizes, int nbins, const float *projections,
                 int nsubsets, int iterations, float beta, float gamma, int nonneg, float *volume_out,

@@ -5,6 +5,8 @@
  * arrays (first index fastest) so that any FFI caller -- and the parity
  * tests -- can run them:
  *   gtt_fdk_reconstruct_3d  FDK_reconstruct_3d (xray/FDK_reconstruct_3d.cpp:24-207)
+ *   gtt_fdk_reconstruct_4d  FDK_reconstruct_4d, McKinnon-Bates 4-D FDK
+ *                           (xray/FDK_reconstruct_4d.cpp:24-177)
  *   gtt_os_sart             osSARTSolver over CBSubsetOperator<cuNDArray>
  *                           (solvers/osSARTSolver.h:33-126)
  *   gtt_os_mom              osMOMSolverD + cuTVPrimalDualOperator over
@@ -37,6 +39,14 @@ int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles_deg, const fl
                            int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
                            const float *projections, const float *filter /* nu+1 or NULL */,
                            float samples_per_pixel, float *volume_out, double *scale_out);
+
+/* volume_out: nx x ny x nz x nbins (one volume per bin, also for nbins ==
+ * 1); fdk3d_out (nullable): the FBP over the union of the bins. */
+int gtt_fdk_reconstruct_4d(const cbct_geom *g, const float *angles_deg, const float *offsets,
+                           int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
+                           const float *projections, const float *filter /* nu+1 or NULL */,
+                           float samples_per_pixel, float *volume_out, float *fdk3d_out,
+                           double *scale_out);
 
 int gtt_os_sart(const cbct_geom *g, const float *angles_deg, const float *offsets, int nviews,
                 const unsigned *bin_index, const int *bin_sizes, int nbins,

@@ -13,6 +13,7 @@ restates the host-side logic of the reference in numpy float64:
 * ``osSARTSolver::solve`` (solvers/osSARTSolver.h:33-126)
 * ``osMOMSolverD::solve`` (solvers/osMOMSolverD.h:82-247, base ``updateG``)
 * the FDK driver (xray/FDK_reconstruct_3d.cpp:128-170)
+* the McKinnon-Bates 4-D FDK driver (xray/FDK_reconstruct_4d.cpp:100-170)
 
 Array layout: volumes are numpy arrays of shape (nz, ny, nx) (x fastest) or
 (nbins, nz, ny, nx); projections are (nviews, nv, nu).
@@ -669,6 +670,31 @@ def fdk_reconstruct_3d(geom: Geom, angles, offsets, proj, F, binning: Binning = 
     p = np.asarray(proj, np.float64)
     scale = float(np.dot(Af.ravel(), p.ravel()) / np.dot(Af.ravel(), Af.ravel()))
     return f * scale, scale
+
+
+def fdk_reconstruct_4d(geom: Geom, angles, offsets, proj, F, binning: Binning,
+                       samples_per_pixel=1.5):
+    """FDK_reconstruct_4d driver (xray/FDK_reconstruct_4d.cpp:100-170), in the
+    reference's own steps: FBP over the 3-D binning (union of the bins, :112,
+    :126-130); the result expanded into every bin and projected bin by bin
+    through the 4-D operator (:145-150; views in no bin stay zero); the
+    least-squares scale (:151); the scaled projection subtracted (:153-154);
+    the 4-D FBP of the residual plus the expanded image, times the scale
+    (:158-166).  Returns (volume (nbins, nz, ny, nx), fdk_3d, scale)."""
+    angles = np.asarray(angles, np.float32)
+    angles = (angles - np.min(angles)).astype(np.float32)
+    if offsets is None:
+        offsets = np.zeros((len(angles), 2), np.float32)
+    b3 = binning.get_3d_binning()
+    f3 = fdk_mult_MH(geom, angles, offsets, proj, b3, F)
+    nb = len(binning)
+    fdk = np.broadcast_to(f3, (nb,) + f3.shape).copy()
+    diff = ho_mult_M(geom, angles, offsets, fdk, binning, samples_per_pixel, use_fbp=True)
+    p = np.asarray(proj, np.float64)
+    scale = float(np.dot(diff.ravel(), p.ravel()) / np.dot(diff.ravel(), diff.ravel()))
+    resid = p - scale * diff
+    res = fdk_mult_MH(geom, angles, offsets, resid, binning, F).reshape(fdk.shape)
+    return (res + fdk) * scale, f3, scale
 
 
 # --------------------------------------------------------------------------

@@ -575,3 +575,76 @@ def test_fdk_scale_cancels_in_driver():
     f2, s2 = O.fdk_reconstruct_3d(g, angles, None, p, 7.0 * F)
     assert np.allclose(f1, f2, rtol=1e-12, atol=1e-15)
     assert s1 == pytest.approx(7.0 * s2, rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# McKinnon-Bates 4-D FDK (xray/FDK_reconstruct_4d.cpp:100-170)
+# ---------------------------------------------------------------------------
+
+def mb_problem(nviews=12, arc=360.0):
+    g = small_geom(nx=6, nu=10, nv=8)
+    angles = (np.arange(nviews, dtype=np.float32) * np.float32(arc / nviews)).astype(np.float32)
+    vol = rand_vol(g, 3)
+    p = O.fp(g, angles, None, vol)
+    # a view-dependent change, so that the bins see different data
+    p *= (1.0 + 0.2 * np.sin(np.arange(nviews)))[:, None, None]
+    return g, angles, p
+
+
+@pytest.mark.parametrize("arc", [360.0, 200.0])
+def test_fdk4d_scale_and_3d_image_are_the_3d_drivers(arc):
+    """When the bins cover every view, A(expand f) is the 3-D projection of f:
+    the 4-D scale and the union FBP are those of FDK_reconstruct_3d."""
+    g, angles, p = mb_problem(arc=arc)
+    bins = O.Binning([[0, 3, 6, 9], [1, 4, 7, 10, 2], [5, 8, 11]])
+    F = O.guo_filter(g.nu)
+    v4, f3, s4 = O.fdk_reconstruct_4d(g, angles, None, p, F, bins)
+    v3, s3 = O.fdk_reconstruct_3d(g, angles, None, p, F)
+    assert s4 == pytest.approx(s3, rel=1e-12)
+    assert np.allclose(f3 * s3, v3, rtol=1e-12, atol=1e-14)
+    assert v4.shape == (3,) + g.vol_shape
+
+
+@pytest.mark.parametrize("arc", [360.0, 200.0])
+def test_fdk4d_bin_weighted_sum_identity(arc):
+    """With bins partitioning the views, sum_b n_b FBP_b(q) = N FBP_all(q)
+    (the pre-filter acts on each view alone, FBP_b carries 2/n_b), so
+    sum_b n_b x_b / N = s (2 f - s FBP_all(A f)).  Uneven bin sizes make a
+    dropped +f, a wrong sign, a 2/N weight or a swapped bin fail.  The
+    tolerance covers the float32 2/n factors the reference uses."""
+    g, angles, p = mb_problem(arc=arc)
+    bins = [[0, 3, 6, 9, 11], [1, 4, 7], [2, 5, 8, 10]]
+    F = O.guo_filter(g.nu)
+    v4, f3, s = O.fdk_reconstruct_4d(g, angles, None, p, F, O.Binning(bins))
+    a0 = (angles - angles.min()).astype(np.float32)
+    off = np.zeros((len(angles), 2), np.float32)
+    Af = O.fp(g, a0, off, f3)
+    fbp_Af = O.fdk_mult_MH(g, a0, off, Af, O.Binning.default_3d(len(angles)), F)
+    lhs = sum(len(b) * v4[i] for i, b in enumerate(bins)) / len(angles)
+    rhs = s * (2.0 * f3 - s * fbp_Af)
+    assert np.abs(lhs - rhs).max() <= 1e-7 * np.abs(rhs).max()
+
+
+def test_fdk4d_views_outside_the_bins_take_no_part():
+    """A view in no bin changes neither the union FBP, the scale nor any bin."""
+    g, angles, p = mb_problem()
+    bins = O.Binning([[0, 2, 4, 6, 8, 10], [1, 3, 5, 9]])      # 7 and 11 in no bin
+    F = O.guo_filter(g.nu)
+    v1, f1, s1 = O.fdk_reconstruct_4d(g, angles, None, p, F, bins)
+    q = p.copy()
+    q[[7, 11]] = 5.0 * np.random.default_rng(1).random(q[[7, 11]].shape)
+    v2, f2, s2 = O.fdk_reconstruct_4d(g, angles, None, q, F, bins)
+    assert np.array_equal(v1, v2) and np.array_equal(f1, f2) and s1 == s2
+
+
+def test_fdk4d_one_bin_of_every_view():
+    """A single bin of all views: x = s (2 f - s FBP(A f)) exactly, and the
+    result differs from the 3-D driver's s f by the second FBP pass."""
+    g, angles, p = mb_problem()
+    F = O.guo_filter(g.nu)
+    v4, f3, s = O.fdk_reconstruct_4d(g, angles, None, p, F, O.Binning.default_3d(len(angles)))
+    a0 = (angles - angles.min()).astype(np.float32)
+    off = np.zeros((len(angles), 2), np.float32)
+    fbp_Af = O.fdk_mult_MH(g, a0, off, O.fp(g, a0, off, f3), O.Binning.default_3d(len(angles)), F)
+    assert np.allclose(v4[0], s * (2.0 * f3 - s * fbp_Af), rtol=1e-10, atol=1e-14)
+    assert not np.allclose(v4[0], s * f3, rtol=1e-3)
