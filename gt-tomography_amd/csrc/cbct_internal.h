@@ -39,6 +39,7 @@ struct cbct_ctx_s {
   DeviceBuffer proj_pad;    // padded projections for the backprojector
   DeviceBuffer red;         // reduction partials
   DeviceBuffer filt_tmp;    // filter staging
+  DeviceBuffer ssim_ws;     // MS-SSIM moment fields and taps
   // filter state
   int filter_nu = 0;
   int filter_engine = 0;

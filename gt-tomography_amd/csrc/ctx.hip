@@ -113,7 +113,7 @@ int cbct_ctx_trim(cbct_ctx ctx) {
   if (!ctx) return CBCT_ERR_ARG;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
-  DeviceBuffer *bufs[] = {&ctx->vol_pad, &ctx->proj_pad, &ctx->red, &ctx->filt_tmp};
+  DeviceBuffer *bufs[] = {&ctx->vol_pad, &ctx->proj_pad, &ctx->red, &ctx->filt_tmp, &ctx->ssim_ws};
   for (auto *b : bufs) {
     if (b->ptr) (void)hipFree(b->ptr);
     b->ptr = nullptr;
@@ -174,7 +174,8 @@ const char *cbct_last_error(cbct_ctx ctx) {
 
 size_t cbct_workspace_bytes(cbct_ctx ctx) {
   if (!ctx) return 0;
-  return ctx->vol_pad.bytes + ctx->proj_pad.bytes + ctx->red.bytes + ctx->filt_tmp.bytes;
+  return ctx->vol_pad.bytes + ctx->proj_pad.bytes + ctx->red.bytes + ctx->filt_tmp.bytes +
+         ctx->ssim_ws.bytes;
 }
 
 int cbct_views_set(cbct_ctx ctx, const float *angles_deg, const float *offsets, int n) {

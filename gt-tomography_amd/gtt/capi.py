@@ -136,6 +136,7 @@ def load():
         "cbct_to_bf16": (I, [P, S, P, P, P]),
         "cbct_from_bf16": (I, [P, S, P, P, P]),
         "cbct_reduce_f64": (I, [P, I, S, P, P, ctypes.POINTER(ctypes.c_double), P]),
+        "cbct_mssim": (I, [P, I, I, I, P, P, F, F, F, I, ctypes.POINTER(ctypes.c_double), P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -143,6 +144,15 @@ def load():
         fn.argtypes = args
     _lib = L
     return L
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_number(v):
+    return isinstance(v, (int, float)) or (hasattr(v, "ndim") and getattr(v, "ndim") == 0)
 
 
 def _ptr(t):
@@ -394,6 +404,22 @@ class Context:
 
     def asum(self, a, stream=None):
         return self.reduce(CBCT_ASUM, a, None, stream)
+
+    def mssim(self, image, reference, sigma=1.5, scales=3, stream=None):
+        """Multi-scale SSIM of a device volume (nz, ny, nx) against a reference
+        volume of the same shape (xray/mssim.cpp:77-121).  sigma: a number or
+        (sx, sy, sz) in voxels."""
+        if image.shape != reference.shape or image.dim() != 3:
+            raise ValueError("mssim: image and reference must be 3-D volumes of one shape")
+        for t in (image, reference):
+            if t.dtype != _torch().float32 or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError("mssim: contiguous float32 device volumes expected")
+        sx, sy, sz = (sigma,) * 3 if _is_number(sigma) else tuple(sigma)
+        nz, ny, nx = image.shape
+        out = ctypes.c_double()
+        self._check(self.lib.cbct_mssim(self.h, nx, ny, nz, _ptr(image), _ptr(reference), sx, sy, sz,
+                                        int(scales), ctypes.byref(out), _stream(stream)))
+        return out.value
 
 
 def cosine_weights(g: Geom):

@@ -38,13 +38,17 @@ def load():
     L.gtt_fdk_reconstruct_4d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, P, D]
     L.gtt_os_sart.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, P, D]
     L.gtt_os_mom.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, F, F, I, I, P]
+    L.gtt_os_sart_weighted.argtypes = [G, P, P, I, P, P, I, P, P, I, I, F, F, I, P, D]
+    L.gtt_os_mom_weighted.argtypes = [G, P, P, I, P, P, I, P, P, I, I, F, F, I, F, F, I, I, P]
+    L.gtt_subset_op_setup_weighted.argtypes = [P, G, P, P, I, P, P, I, P, P, P, P]
     L.gtt_subset_op_create.argtypes = [I, ctypes.POINTER(P)]
     L.gtt_subset_op_destroy.argtypes = [P]
     L.gtt_subset_op_setup.argtypes = [P, G, P, P, I, P, P, I, P, P, P]
     L.gtt_subset_op_views.argtypes = [P, I]
     L.gtt_subset_op_mult_M.argtypes = [P, P, P, I, I]
     L.gtt_subset_op_mult_MH.argtypes = [P, P, P, I, I]
-    for n in ("gtt_fdk_reconstruct_3d", "gtt_fdk_reconstruct_4d", "gtt_os_sart", "gtt_os_mom", "gtt_subset_op_create",
+    for n in ("gtt_fdk_reconstruct_3d", "gtt_fdk_reconstruct_4d", "gtt_os_sart_weighted",
+              "gtt_os_mom_weighted", "gtt_subset_op_setup_weighted", "gtt_os_sart", "gtt_os_mom", "gtt_subset_op_create",
               "gtt_subset_op_destroy", "gtt_subset_op_setup", "gtt_subset_op_views",
               "gtt_subset_op_mult_M", "gtt_subset_op_mult_MH"):
         getattr(L, n).restype = I
@@ -125,32 +129,48 @@ def fdk_reconstruct_4d(g: capi.Geom, angles, offsets, projections, bins, filt=No
     return out, f3, scale.value
 
 
+def _weights(weights, p):
+    if weights is None:
+        return None
+    w = np.ascontiguousarray(weights, np.float32)
+    if w.shape != p.shape:
+        raise ValueError("Weight dimensions must match that of the projection data")
+    return w
+
+
 def os_sart(g: capi.Geom, angles, offsets, projections, nsubsets, iterations, beta=1.0, gamma=0.0,
-            nonneg=False, bins=None, objective=False):
-    """osSARTSolver over CBSubsetOperator; projections in acquisition order."""
+            nonneg=False, bins=None, objective=False, weights=None):
+    """osSARTSolver over CBSubsetOperator (CBSubsetWeightOperator when
+    per-pixel weights of the projections' shape are given); projections in
+    acquisition order."""
     L = load()
     a, o = _views(angles, offsets)
     idx, sizes, nb = _bins(bins)
     p = np.ascontiguousarray(projections, np.float32)
+    w = _weights(weights, p)
     out = np.zeros(_vol_shape(g, nb), np.float32)
     obj = np.zeros(iterations, np.float64) if objective else None
-    _check(L.gtt_os_sart(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb, _p(p),
-                         nsubsets, iterations, beta, gamma, int(nonneg), _p(out),
-                         None if obj is None else obj.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    _check(L.gtt_os_sart_weighted(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb,
+                                  _p(p), _p(w), nsubsets, iterations, beta, gamma, int(nonneg),
+                                  _p(out), None if obj is None else
+                                  obj.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     return (out, obj) if objective else out
 
 
 def os_mom(g: capi.Geom, angles, offsets, projections, nsubsets, iterations, tv_weight=0.0,
-           tau=1e-3, reg_steps=1, beta=1.0, gamma=0.0, nonneg=False, fuse_tv=True, bins=None):
-    """osMOMSolverD (+ isotropic TV) over CBSubsetOperator; returns z."""
+           tau=1e-3, reg_steps=1, beta=1.0, gamma=0.0, nonneg=False, fuse_tv=True, bins=None,
+           weights=None):
+    """osMOMSolverD (+ isotropic TV) over CBSubsetOperator, or
+    CBSubsetWeightOperator with per-pixel weights; returns z."""
     L = load()
     a, o = _views(angles, offsets)
     idx, sizes, nb = _bins(bins)
     p = np.ascontiguousarray(projections, np.float32)
+    w = _weights(weights, p)
     out = np.zeros(_vol_shape(g, nb), np.float32)
-    _check(L.gtt_os_mom(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb, _p(p),
-                        nsubsets, iterations, tv_weight, tau, reg_steps, beta, gamma, int(nonneg),
-                        int(fuse_tv), _p(out)))
+    _check(L.gtt_os_mom_weighted(ctypes.byref(g), _p(a), _p(o), a.size, _p(idx), _p(sizes), nb,
+                                 _p(p), _p(w), nsubsets, iterations, tv_weight, tau, reg_steps,
+                                 beta, gamma, int(nonneg), int(fuse_tv), _p(out)))
     return out
 
 
@@ -171,17 +191,23 @@ class CBSubsetOperator:
         except Exception:
             pass
 
-    def setup(self, g: capi.Geom, angles, offsets, projections=None, bins=None):
-        """Returns (permuted projections or None, permutation)."""
+    def setup(self, g: capi.Geom, angles, offsets, projections=None, bins=None, weights=None):
+        """Returns (permuted projections or None, permutation).  With weights
+        (the projections' shape) the operator is a CBSubsetWeightOperator and
+        the returned projections are the weighted ones."""
         self.g = g
         a, o = _views(angles, offsets)
         idx, sizes, nb = _bins(bins)
         self.nbins = max(nb, 1)
         p = None if projections is None else np.ascontiguousarray(projections, np.float32)
+        if weights is not None and p is None:
+            raise ValueError("weighted setup needs the projections")
+        w = None if weights is None else _weights(weights, p)
         perm_p = None if p is None else np.empty_like(p)
         perm = np.zeros(a.size, np.uint32)
-        _check(self.L.gtt_subset_op_setup(self.h, ctypes.byref(g), _p(a), _p(o), a.size, _p(idx),
-                                          _p(sizes), nb, _p(p), _p(perm_p), _p(perm)))
+        _check(self.L.gtt_subset_op_setup_weighted(self.h, ctypes.byref(g), _p(a), _p(o), a.size,
+                                                   _p(idx), _p(sizes), nb, _p(p), _p(w),
+                                                   _p(perm_p), _p(perm)))
         return perm_p, perm
 
     def views(self, subset=-1):

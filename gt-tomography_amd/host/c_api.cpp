@@ -38,6 +38,23 @@ int guarded(F &&f) {
   }
 }
 
+// CBSubsetOperator, or CBSubsetWeightOperator when weights (host, the
+// projections' shape) are given, set up on the scan (cuCBOS_reconstruct.cpp:481-503).
+std::shared_ptr<CBSubsetOperator> make_subset_operator(int nsubsets, const Scan &s, const float *weights) {
+  if (!weights) {
+    auto E = std::make_shared<CBSubsetOperator>(nsubsets);
+    E->setup(s.acq, s.binning, s.dims_mm);
+    return E;
+  }
+  auto proj = s.acq->get_projections();
+  if (!proj) throw std::runtime_error("Weight dimensions must match that of the projection data");
+  auto w = std::make_shared<hoCuNDArray<float>>(proj->dims());
+  std::memcpy(w->get_data_ptr(), weights, w->get_number_of_bytes());
+  auto E = std::make_shared<CBSubsetWeightOperator>(nsubsets);
+  E->setup(s.acq, s.binning, s.dims_mm, w);
+  return E;
+}
+
 }  // namespace
 
 struct gtt_subset_op_s {
@@ -93,12 +110,21 @@ int gtt_os_sart(const cbct_geom *g, const float *angles, const float *offsets, i
                 const unsigned *bin_index, const int *bin_sizes, int nbins, const float *projections,
                 int nsubsets, int iterations, float beta, float gamma, int nonneg, float *volume_out,
                 double *objective_out) {
+  return gtt_os_sart_weighted(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections,
+                              nullptr, nsubsets, iterations, beta, gamma, nonneg, volume_out,
+                              objective_out);
+}
+
+int gtt_os_sart_weighted(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
+                         const unsigned *bin_index, const int *bin_sizes, int nbins,
+                         const float *projections, const float *weights, int nsubsets,
+                         int iterations, float beta, float gamma, int nonneg, float *volume_out,
+                         double *objective_out) {
   return guarded([&] {
     if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
     if (nsubsets <= 0) throw std::runtime_error("number of subsets must be positive");
     Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
-    auto E = std::make_shared<CBSubsetOperator>(nsubsets);
-    E->setup(s.acq, s.binning, s.dims_mm);
+    auto E = make_subset_operator(nsubsets, s, weights);
     E->set_domain_dimensions(&s.vol_dims);
     cuNDArray<float> b(*s.acq->get_projections());
     osSARTSolver<cuNDArray<float>> solver;
@@ -118,12 +144,21 @@ int gtt_os_mom(const cbct_geom *g, const float *angles, const float *offsets, in
                const unsigned *bin_index, const int *bin_sizes, int nbins, const float *projections,
                int nsubsets, int iterations, float tv_weight, float tau, int reg_steps, float beta,
                float gamma, int nonneg, int fuse_tv, float *volume_out) {
+  return gtt_os_mom_weighted(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections,
+                             nullptr, nsubsets, iterations, tv_weight, tau, reg_steps, beta, gamma,
+                             nonneg, fuse_tv, volume_out);
+}
+
+int gtt_os_mom_weighted(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
+                        const unsigned *bin_index, const int *bin_sizes, int nbins,
+                        const float *projections, const float *weights, int nsubsets,
+                        int iterations, float tv_weight, float tau, int reg_steps, float beta,
+                        float gamma, int nonneg, int fuse_tv, float *volume_out) {
   return guarded([&] {
     if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
     if (nsubsets <= 0) throw std::runtime_error("number of subsets must be positive");
     Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
-    auto E = std::make_shared<CBSubsetOperator>(nsubsets);
-    E->setup(s.acq, s.binning, s.dims_mm);
+    auto E = make_subset_operator(nsubsets, s, weights);
     E->set_domain_dimensions(&s.vol_dims);
     cuNDArray<float> b(*s.acq->get_projections());
     osMOMSolverD<cuNDArray<float>> solver;
@@ -164,13 +199,28 @@ int gtt_subset_op_setup(gtt_subset_op op, const cbct_geom *g, const float *angle
                         const float *offsets, int nviews, const unsigned *bin_index,
                         const int *bin_sizes, int nbins, const float *projections,
                         float *permuted_out, unsigned *permutation_out) {
+  return gtt_subset_op_setup_weighted(op, g, angles, offsets, nviews, bin_index, bin_sizes, nbins,
+                                      projections, nullptr, permuted_out, permutation_out);
+}
+
+int gtt_subset_op_setup_weighted(gtt_subset_op op, const cbct_geom *g, const float *angles,
+                                 const float *offsets, int nviews, const unsigned *bin_index,
+                                 const int *bin_sizes, int nbins, const float *projections,
+                                 const float *weights, float *permuted_out,
+                                 unsigned *permutation_out) {
   return guarded([&] {
     if (!op) throw std::runtime_error("null operator");
     Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
     if (!projections)  // projection dims still come from the geometry
       s.acq->set_projections(std::make_shared<hoCuNDArray<float>>(
           std::vector<size_t>{(size_t)g->nu, (size_t)g->nv, (size_t)nviews}));
-    op->op->setup(s.acq, s.binning, s.dims_mm);
+    if (weights) {
+      op->op = make_subset_operator(op->op->get_number_of_subsets(), s, weights);
+    } else {
+      if (std::dynamic_pointer_cast<CBSubsetWeightOperator>(op->op))
+        op->op = std::make_shared<CBSubsetOperator>(op->op->get_number_of_subsets());
+      op->op->setup(s.acq, s.binning, s.dims_mm);
+    }
     op->op->set_domain_dimensions(&s.vol_dims);
     auto p = s.acq->get_projections();
     if (permuted_out && projections) std::memcpy(permuted_out, p->get_data_ptr(), p->get_number_of_bytes());

@@ -358,6 +358,60 @@ void CBSubsetOperator::offset_correct(cuNDArray<float> *projections) {
 }
 
 // ---------------------------------------------------------------------------
+void CBSubsetWeightOperator::setup(std::shared_ptr<CBCT_acquisition> acq,
+                                   std::shared_ptr<CBCT_binning> binning, floatd3 is_dims_in_mm,
+                                   std::shared_ptr<hoCuNDArray<float>> projection_weights) {
+  auto proj = acq->get_projections();
+  if (!proj || !projection_weights || proj->dims() != projection_weights->dims())
+    throw std::runtime_error("Weight dimensions must match that of the projection data");
+  {
+    float *p = proj->get_data_ptr();
+    const float *w = projection_weights->get_data_ptr();
+    for (size_t i = 0, n = proj->get_number_of_elements(); i < n; ++i) p[i] *= w[i];
+  }
+  CBSubsetOperator::setup(acq, binning, is_dims_in_mm);
+  // the weights in the projections' permuted order (permute_projections)
+  auto dims = projection_weights->dims();
+  dims.back() = permutations_.size();
+  hoCuNDArray<float> permuted(dims);
+  const size_t plane = projection_weights->get_size(0) * projection_weights->get_size(1);
+  for (size_t i = 0; i < permutations_.size(); ++i)
+    std::copy(projection_weights->get_data_ptr() + plane * permutations_[i],
+              projection_weights->get_data_ptr() + plane * (permutations_[i] + 1),
+              permuted.get_data_ptr() + plane * i);
+  weights_ = cuNDArray<float>(permuted);
+  subset_weights_ = projection_subsets(&weights_);
+}
+
+void CBSubsetWeightOperator::mult_M(cuNDArray<float> *in, cuNDArray<float> *out, int subset, bool acc) {
+  if (subset < 0 || subset >= (int)subset_weights_.size())
+    throw std::runtime_error("CBSubsetWeightOperator::mult_M: setup not performed or subset out of range");
+  if (acc) {
+    cuNDArray<float> tmp(out->dims());
+    CBSubsetOperator::mult_M(in, &tmp, subset, false);
+    tmp *= *subset_weights_[subset];
+    *out += tmp;
+  } else {
+    CBSubsetOperator::mult_M(in, out, subset, false);
+    *out *= *subset_weights_[subset];
+  }
+}
+
+void CBSubsetWeightOperator::mult_MH(cuNDArray<float> *in, cuNDArray<float> *out, int subset, bool acc) {
+  if (subset < 0 || subset >= (int)subset_weights_.size())
+    throw std::runtime_error("CBSubsetWeightOperator::mult_MH: setup not performed or subset out of range");
+  cuNDArray<float> tmp_in(*in);
+  tmp_in *= *subset_weights_[subset];
+  CBSubsetOperator::mult_MH(&tmp_in, out, subset, acc);
+}
+
+void CBSubsetWeightOperator::mult_MH_M(cuNDArray<float> *in, cuNDArray<float> *out, int subset, bool acc) {
+  cuNDArray<float> tmp(get_codomain_dimensions(subset));
+  mult_M(in, &tmp, subset, false);
+  mult_MH(&tmp, out, subset, acc);
+}
+
+// ---------------------------------------------------------------------------
 void cuTVPrimalDualOperator::primalDual(cuNDArray<float> *in, cuNDArray<float> *out, float sigma,
                                         bool accumulate) {
   if (!in->dimensions_equal(out)) throw std::runtime_error("Input and reference dimensions must agree");

@@ -258,6 +258,20 @@ enum { CBCT_DOT = 0, CBCT_NRM2 = 1, CBCT_ASUM = 2, CBCT_SUM = 3 };
 int cbct_reduce_f64(cbct_ctx ctx, int op, size_t n, const float *a, const float *b,
                     double *out_host, void *stream);
 
+/* ----------------------------------------------------------- image quality */
+/* Multi-scale SSIM of a volume against a reference volume (device arrays,
+ * nx*ny*nz, x fastest), written to *out_host:
+ *   mean( prod_{i<scales} CS(sigma 2^i) * L(sigma 2^(scales-1)) ),
+ *   CS = (2 cov + C2)/(var_x + var_y + C2), L = (2 mu_x mu_y + C1)/(mu_x^2 + mu_y^2 + C1),
+ *   C1 = (0.01 R)^2, C2 = (0.03 R)^2, R = max(reference) - min(reference).
+ * Local moments are Gaussian-weighted: separable taps exp(-k^2/(2 sigma^2)),
+ * |k| <= int(3.5 sigma + 0.5), normalised; half-sample symmetric borders.
+ * Workspace: 11 floats per voxel (context-owned).  Synchronises the stream.
+ * Replaces Gadgetron::mssim (xray/mssim.h:7, xray/mssim.cpp:77-121). */
+int cbct_mssim(cbct_ctx ctx, int nx, int ny, int nz, const float *image, const float *reference,
+               float sigma_x, float sigma_y, float sigma_z, int scales, double *out_host,
+               void *stream);
+
 #ifdef __cplusplus
 }
 #endif

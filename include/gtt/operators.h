@@ -263,6 +263,36 @@ class CBSubsetOperator : public subsetOperator<cuNDArray<float>> {
 };
 
 // ---------------------------------------------------------------------------
+// Per-projection statistical weights around the subset projector
+// (xray/CBSubsetWeightOperator.h:11-63): M_w = W A, M_w^H = A^H W, with W
+// the weights in the subsets' permuted order.
+// ---------------------------------------------------------------------------
+class CBSubsetWeightOperator : public CBSubsetOperator {
+ public:
+  explicit CBSubsetWeightOperator(int subsets = 1) : CBSubsetOperator(subsets) {}
+  // :17-31 -- the acquisition's projections are multiplied by the weights
+  // (in place, host), the subsets are set up, and the weights are permuted
+  // like the projections and kept on the device.
+  void setup(std::shared_ptr<CBCT_acquisition> acq, std::shared_ptr<CBCT_binning> binning,
+             floatd3 is_dims_in_mm, std::shared_ptr<hoCuNDArray<float>> projection_weights);
+  using CBSubsetOperator::setup;
+  // :33-43 -- the subset's projection times its weights
+  void mult_M(cuNDArray<float> *in, cuNDArray<float> *out, int subset, bool accumulate) override;
+  // :44-49 -- the input times the subset's weights, backprojected
+  void mult_MH(cuNDArray<float> *in, cuNDArray<float> *out, int subset, bool accumulate) override;
+  // :50-57
+  void mult_MH_M(cuNDArray<float> *in, cuNDArray<float> *out, int subset, bool accumulate) override;
+  using CBSubsetOperator::mult_M;
+  using CBSubsetOperator::mult_MH;
+  using CBSubsetOperator::mult_MH_M;
+  const cuNDArray<float> &get_weights() const { return weights_; }
+
+ protected:
+  cuNDArray<float> weights_;
+  std::vector<std::shared_ptr<cuNDArray<float>>> subset_weights_;
+};
+
+// ---------------------------------------------------------------------------
 // Primal-dual regularisers (operators/primalDualOperator.h,
 // operators/cuTVPrimalDualOperator.cu)
 // ---------------------------------------------------------------------------

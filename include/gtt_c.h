@@ -11,6 +11,8 @@
  *                           (solvers/osSARTSolver.h:33-126)
  *   gtt_os_mom              osMOMSolverD + cuTVPrimalDualOperator over
  *                           CBSubsetOperator<cuNDArray> (xray/cuCBOS_reconstruct.cpp:271-523)
+ *   gtt_*_weighted          the same over CBSubsetWeightOperator<cuNDArray>
+ *                           (xray/CBSubsetWeightOperator.h)
  *   gtt_subset_op_*         CBSubsetOperator<cuNDArray> (xray/CBSubsetOperator.cpp)
  *                           on device pointers, for operator-level checks.
  * Binning: bin_index holds the concatenated view indices of nbins bins whose
@@ -59,6 +61,21 @@ int gtt_os_mom(const cbct_geom *g, const float *angles_deg, const float *offsets
                float tau, int reg_steps, float beta, float gamma, int nonneg, int fuse_tv,
                float *volume_out);
 
+/* The same solvers over CBSubsetWeightOperator (xray/CBSubsetWeightOperator.h:11-63,
+ * selected by --projection_weights in xray/cuCBOS_reconstruct.cpp:481-498):
+ * weights (host, the projections' shape) multiply the data and both sides of
+ * the operator.  weights == NULL is the unweighted solver. */
+int gtt_os_sart_weighted(const cbct_geom *g, const float *angles_deg, const float *offsets,
+                         int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
+                         const float *projections, const float *weights, int nsubsets,
+                         int iterations, float beta, float gamma, int nonneg, float *volume_out,
+                         double *objective_out);
+int gtt_os_mom_weighted(const cbct_geom *g, const float *angles_deg, const float *offsets,
+                        int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
+                        const float *projections, const float *weights, int nsubsets,
+                        int iterations, float tv_weight, float tau, int reg_steps, float beta,
+                        float gamma, int nonneg, int fuse_tv, float *volume_out);
+
 typedef struct gtt_subset_op_s *gtt_subset_op;
 int gtt_subset_op_create(int nsubsets, gtt_subset_op *out);
 int gtt_subset_op_destroy(gtt_subset_op op);
@@ -68,6 +85,14 @@ int gtt_subset_op_setup(gtt_subset_op op, const cbct_geom *g, const float *angle
                         const float *offsets, int nviews, const unsigned *bin_index,
                         const int *bin_sizes, int nbins, const float *projections,
                         float *permuted_out, unsigned *permutation_out);
+/* As gtt_subset_op_setup, as a CBSubsetWeightOperator when weights (host,
+ * the projections' shape) are given; permuted_out then holds the weighted
+ * projections. */
+int gtt_subset_op_setup_weighted(gtt_subset_op op, const cbct_geom *g, const float *angles_deg,
+                                 const float *offsets, int nviews, const unsigned *bin_index,
+                                 const int *bin_sizes, int nbins, const float *projections,
+                                 const float *weights, float *permuted_out,
+                                 unsigned *permutation_out);
 int gtt_subset_op_views(gtt_subset_op op, int subset);
 /* subset < 0: the full operator.  Device pointers, default stream. */
 int gtt_subset_op_mult_M(gtt_subset_op op, float *vol_dev, float *proj_dev, int subset, int accumulate);

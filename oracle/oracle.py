@@ -14,6 +14,12 @@ restates the host-side logic of the reference in numpy float64:
 * ``osMOMSolverD::solve`` (solvers/osMOMSolverD.h:82-247, base ``updateG``)
 * the FDK driver (xray/FDK_reconstruct_3d.cpp:128-170)
 * the McKinnon-Bates 4-D FDK driver (xray/FDK_reconstruct_4d.cpp:100-170)
+* ``CBSubsetWeightOperator`` (xray/CBSubsetWeightOperator.h:11-63)
+* multi-scale SSIM (xray/mssim.cpp:14-121) over a Gaussian filter restated
+  from its definition (the reference's filter, Gadgetron's
+  cuGaussianFilterOperator, is not in the reference sources and no version
+  of it is pinned there: its tap truncation and border rule are parity
+  unpinned; the convention here is fixed by the tests against scipy)
 
 Array layout: volumes are numpy arrays of shape (nz, ny, nx) (x fastest) or
 (nbins, nz, ny, nx); projections are (nviews, nv, nu).
@@ -493,6 +499,38 @@ class CBSubsetOperator:
             self.mult_MH_subset(subs[i], out, i, True)
 
 
+class CBSubsetWeightOperator(CBSubsetOperator):
+    """CBSubsetWeightOperator (xray/CBSubsetWeightOperator.h:11-63): setup
+    multiplies the projections by the weights (the reference does so in the
+    acquisition, in place, :20) before the subset split, and keeps the
+    weights in the same permuted order; mult_M weights a subset's projection
+    (:33-43), mult_MH weights its input before backprojecting (:44-49)."""
+
+    def setup(self, geom: Geom, angles, offsets, projections, binning: Binning, weights,
+              nbins_dom=None, samples_per_pixel=1.5):
+        w = np.asarray(weights, np.float64)
+        p = np.asarray(projections, np.float64)
+        if w.shape != p.shape:
+            raise ValueError("Weight dimensions must match that of the projection data")
+        pb = super().setup(geom, angles, offsets, p * w, binning, nbins_dom, samples_per_pixel)
+        self.weights = np.ascontiguousarray(w[self.perm])
+        self.subset_weights = self.projection_subsets(self.weights)
+        return pb
+
+    def mult_M_subset(self, x, out, subset, accumulate=False):
+        tmp = np.zeros(out.shape)
+        super().mult_M_subset(x, tmp, subset, False)
+        tmp *= self.subset_weights[subset]
+        if accumulate:
+            out += tmp
+        else:
+            out[...] = tmp
+
+    def mult_MH_subset(self, y, out, subset, accumulate=False):
+        super().mult_MH_subset(np.asarray(y, np.float64) * self.subset_weights[subset], out, subset,
+                               accumulate)
+
+
 # --------------------------------------------------------------------------
 # Solvers
 # --------------------------------------------------------------------------
@@ -695,6 +733,70 @@ def fdk_reconstruct_4d(geom: Geom, angles, offsets, proj, F, binning: Binning,
     resid = p - scale * diff
     res = fdk_mult_MH(geom, angles, offsets, resid, binning, F).reshape(fdk.shape)
     return (res + fdk) * scale, f3, scale
+
+
+# --------------------------------------------------------------------------
+# Multi-scale SSIM
+# --------------------------------------------------------------------------
+
+def gauss_taps(sigma, truncate=3.5):
+    """Normalised taps exp(-k^2/(2 sigma^2)), |k| <= int(truncate sigma + 0.5)."""
+    sigma = float(sigma)
+    if sigma <= 0.0:
+        return np.ones(1)
+    r = int(truncate * sigma + 0.5)
+    k = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 * k * k / (sigma * sigma))
+    return w / w.sum()
+
+
+def _reflect(i, n):
+    """Half-sample symmetric index (-1 -> 0, n -> n-1), any distance."""
+    period = 2 * n
+    i = np.mod(i, period)
+    return np.where(i < n, i, period - 1 - i)
+
+
+def gauss_filter(vol, sigma_xyz, truncate=3.5):
+    """Separable Gaussian filter of a (nz, ny, nx) array; sigma per (x, y, z)."""
+    out = np.asarray(vol, np.float64)
+    for axis, sigma in ((2, sigma_xyz[0]), (1, sigma_xyz[1]), (0, sigma_xyz[2])):
+        w = gauss_taps(sigma, truncate)
+        r = (w.size - 1) // 2
+        n = out.shape[axis]
+        acc = np.zeros_like(out)
+        for j, wk in enumerate(w):
+            idx = _reflect(np.arange(n) + (j - r), n)
+            acc += wk * np.take(out, idx, axis=axis)
+        out = acc
+    return out
+
+
+def _cs_map(X, Y, sigma, c2):
+    """CS (xray/mssim.cpp:14-74): (2 cov + C2) / (var_x + var_y + C2)."""
+    ux, uy = gauss_filter(X, sigma), gauss_filter(Y, sigma)
+    xx, yy, xy = gauss_filter(X * X, sigma), gauss_filter(Y * Y, sigma), gauss_filter(X * Y, sigma)
+    return (2.0 * (xy - ux * uy) + c2) / ((xx - ux * ux) + (yy - uy * uy) + c2)
+
+
+def mssim(image, reference, sigma_xyz=(1.5, 1.5, 1.5), scales=3):
+    """Gadgetron::mssim (xray/mssim.cpp:77-121) in float64: the CS maps at
+    sigma 2^i for i < scales multiplied together, times the luminance term at
+    sigma 2^(scales-1), averaged.  C1 = (0.01 R)^2, C2 = (0.03 R)^2 with R
+    the reference's range, formed in float32 as the reference does."""
+    X = np.asarray(image, np.float64)
+    Y = np.asarray(reference, np.float64)
+    rng = np.float32(np.float32(Y.max()) - np.float32(Y.min()))
+    c1 = float(np.float32(np.float64(np.float32(0.01) * rng) ** 2))
+    c2 = float(np.float32(np.float64(np.float32(0.03) * rng) ** 2))
+    sig = np.asarray(sigma_xyz, np.float32)
+    S = np.ones_like(X)
+    for i in range(scales):
+        S *= _cs_map(X, Y, sig * np.float32(2.0 ** i), c2)
+    top = sig * np.float32(2.0 ** (scales - 1))
+    ux, uy = gauss_filter(X, top), gauss_filter(Y, top)
+    S *= (2.0 * ux * uy + c1) / (ux * ux + uy * uy + c1)
+    return float(S.mean())
 
 
 # --------------------------------------------------------------------------

@@ -101,6 +101,85 @@ def test_subset_operator_two_bins():
     assert rel_l2(v.cpu().numpy(), refv) <= 1e-5
 
 
+def stat_weights(p, seed=11):
+    """Per-pixel weights in [0.3, 1.7] of the projections' shape, with one view weighted zero."""
+    w = np.random.default_rng(seed).uniform(0.3, 1.7, p.shape).astype(np.float32)
+    w[3] = 0.0
+    return w
+
+
+def test_weighted_subset_operator_matches_oracle():
+    """CBSubsetWeightOperator (xray/CBSubsetWeightOperator.h:17-57): weighted
+    data, weighted projections, weighted backprojection input."""
+    g, angles, offs, vol, p = problem(offsets=True)
+    w = stat_weights(p)
+    S = 4
+    print("CASE weighted subset-op", flush=True)
+    gop = G.CBSubsetOperator(S)
+    perm_p, perm = gop.setup(g, angles, offs, p, weights=w)
+    oop = O.CBSubsetWeightOperator(S)
+    pbw = oop.setup(to_oracle(g), angles, offs, p, O.Binning.default_3d(len(angles)), w)
+    assert list(perm) == oop.perm
+    assert np.array_equal(perm_p, (p * w)[oop.perm])          # float32 product on the host
+    x = torch.from_numpy(vol.astype(np.float32)).cuda()
+    y = torch.zeros(p.shape, device="cuda")
+    gop.mult_M(x, y)
+    torch.cuda.synchronize()
+    ref = np.zeros(p.shape)
+    oop.mult_M(vol, ref)
+    assert rel_l2(y.cpu().numpy(), ref) <= 1e-5
+    s = 1
+    n0 = sum(c[0] for c in oop.codomain[:s])
+    ns = oop.codomain[s][0]
+    ys = torch.ones((ns, g.nv, g.nu), device="cuda")
+    gop.mult_M(x, ys, subset=s, accumulate=True)
+    torch.cuda.synchronize()
+    assert rel_l2(ys.cpu().numpy(), 1.0 + ref[n0:n0 + ns]) <= 1e-5
+    v = torch.zeros_like(x)
+    gop.mult_MH(torch.from_numpy(perm_p).cuda(), v)
+    torch.cuda.synchronize()
+    refv = np.zeros(vol.shape)
+    oop.mult_MH(pbw.astype(np.float64), refv)
+    assert rel_l2(v.cpu().numpy(), refv) <= 1e-5
+    # the same handle set up again without weights is the plain operator
+    gop.setup(g, angles, offs, p)
+    y2 = torch.zeros(p.shape, device="cuda")
+    gop.mult_M(x, y2)
+    torch.cuda.synchronize()
+    plain = np.zeros(p.shape)
+    oplain, _ = oracle_subset_op(g, angles, offs, p, S)
+    oplain.mult_M(vol, plain)
+    assert rel_l2(y2.cpu().numpy(), plain) <= 1e-5
+
+
+def test_weighted_solvers_match_oracle():
+    """osMOMSolverD (+TV) and OS-SART over CBSubsetWeightOperator
+    (xray/cuCBOS_reconstruct.cpp:481-498)."""
+    g, angles, offs, vol, p = problem(noise=True, seed=6)
+    w = stat_weights(p, seed=12)
+    S, iters = 3, 3
+    print("CASE weighted os-mom / os-sart", flush=True)
+    oop = O.CBSubsetWeightOperator(S)
+    o = np.zeros((len(angles), 2), np.float32)
+    pbw = oop.setup(to_oracle(g), angles, o, p, O.Binning.default_3d(len(angles)), w)
+    z = G.os_mom(g, angles, offs, p, S, iters, tv_weight=1e-3, tau=1e-3, reg_steps=2, nonneg=True,
+                 weights=w)
+    ref = O.os_mom(oop, pbw, iters, tv_weight=1e-3, tau0=1e-3, reg_steps=2, nonneg=True)
+    err = rel_l2(z, ref)
+    print("weighted os-mom rel-L2", err)
+    assert err <= 1e-4
+    x = G.os_sart(g, angles, offs, p, S, iters, beta=0.9, nonneg=True, weights=w)
+    refx = O.os_sart(oop, pbw, iters, beta=0.9, nonneg=True)
+    err = rel_l2(x, refx)
+    print("weighted os-sart rel-L2", err)
+    assert err <= 1e-4
+    # the weights change the result
+    assert rel_l2(z, G.os_mom(g, angles, offs, p, S, iters, tv_weight=1e-3, tau=1e-3, reg_steps=2,
+                              nonneg=True)) > 1e-3
+    with pytest.raises(ValueError, match="Weight dimensions"):
+        G.os_mom(g, angles, offs, p, S, 1, weights=w[:-1])
+
+
 @pytest.mark.parametrize("nonneg", [False, True])
 def test_os_sart_matches_oracle(nonneg):
     g, angles, offs, vol, p = problem(noise=True)

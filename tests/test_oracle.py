@@ -648,3 +648,148 @@ def test_fdk4d_one_bin_of_every_view():
     fbp_Af = O.fdk_mult_MH(g, a0, off, O.fp(g, a0, off, f3), O.Binning.default_3d(len(angles)), F)
     assert np.allclose(v4[0], s * (2.0 * f3 - s * fbp_Af), rtol=1e-10, atol=1e-14)
     assert not np.allclose(v4[0], s * f3, rtol=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# Multi-scale SSIM (xray/mssim.cpp:14-121)
+# ---------------------------------------------------------------------------
+
+def scipy_gauss(v, sigma_xyz):
+    return ndimage.gaussian_filter(np.asarray(v, np.float64), sigma=tuple(sigma_xyz)[::-1],
+                                   mode="reflect", truncate=3.5)
+
+
+@pytest.mark.parametrize("sigma", [(1.5, 1.5, 1.5), (0.7, 2.0, 3.3), (0.0, 1.0, 6.0)])
+def test_gauss_filter_is_scipys(sigma):
+    """The oracle's separable filter is scipy.ndimage.gaussian_filter with
+    truncate=3.5 and mode='reflect', also where the radius exceeds the axis
+    (z: 5 voxels against 21 taps a side at sigma 6)."""
+    v = np.random.default_rng(4).random((5, 9, 12))
+    assert np.allclose(O.gauss_filter(v, sigma), scipy_gauss(v, sigma), rtol=0, atol=1e-13)
+    # the 11-tap window of SSIM at sigma 1.5
+    assert O.gauss_taps(1.5).size == 11 and abs(O.gauss_taps(1.5).sum() - 1.0) < 1e-15
+
+
+def ssim_parts_scipy(X, Y, sigma, c1, c2):
+    ux, uy = scipy_gauss(X, sigma), scipy_gauss(Y, sigma)
+    vx = scipy_gauss(X * X, sigma) - ux ** 2
+    vy = scipy_gauss(Y * Y, sigma) - uy ** 2
+    cxy = scipy_gauss(X * Y, sigma) - ux * uy
+    lum = (2 * ux * uy + c1) / (ux ** 2 + uy ** 2 + c1)
+    cs = (2 * cxy + c2) / (vx + vy + c2)
+    return lum, cs
+
+
+def mssim_pair(seed=5, shape=(10, 12, 14)):
+    r = np.random.default_rng(seed)
+    Y = ndimage.gaussian_filter(r.random(shape), 1.0)
+    X = Y + 0.05 * r.standard_normal(shape)
+    return X, Y
+
+
+def test_mssim_one_scale_is_textbook_ssim():
+    """scales = 1 is the mean SSIM of Wang et al. (2004) with a Gaussian
+    window, C1 = (0.01 R)^2, C2 = (0.03 R)^2, R the reference's range."""
+    X, Y = mssim_pair()
+    R = Y.max() - Y.min()
+    lum, cs = ssim_parts_scipy(X, Y, (1.5,) * 3, (0.01 * R) ** 2, (0.03 * R) ** 2)
+    assert O.mssim(X, Y, (1.5,) * 3, scales=1) == pytest.approx(float((lum * cs).mean()), rel=1e-6)
+
+
+def test_mssim_scales_compose_cs_and_top_luminance():
+    """scales = 3: CS at sigma, 2 sigma, 4 sigma and the luminance at 4 sigma."""
+    X, Y = mssim_pair(shape=(12, 12, 16))
+    R = Y.max() - Y.min()
+    c1, c2 = (0.01 * R) ** 2, (0.03 * R) ** 2
+    S = np.ones_like(X)
+    for f in (1.0, 2.0, 4.0):
+        lum, cs = ssim_parts_scipy(X, Y, (0.8 * f, 1.0 * f, 1.2 * f), c1, c2)
+        S *= cs
+    S *= lum
+    assert O.mssim(X, Y, (0.8, 1.0, 1.2), scales=3) == pytest.approx(float(S.mean()), rel=1e-6)
+
+
+def test_mssim_invariants():
+    X, Y = mssim_pair()
+    assert O.mssim(Y, Y) == pytest.approx(1.0, abs=1e-12)
+    # C1, C2 follow the reference's range: a common positive scale cancels
+    assert O.mssim(3.0 * X, 3.0 * Y) == pytest.approx(O.mssim(X, Y), rel=1e-6)
+    # a worse image scores lower
+    r = np.random.default_rng(9)
+    assert O.mssim(Y + 0.2 * r.standard_normal(Y.shape), Y) < O.mssim(X, Y) < 1.0
+
+
+# ---------------------------------------------------------------------------
+# CBSubsetWeightOperator (xray/CBSubsetWeightOperator.h:11-63)
+# ---------------------------------------------------------------------------
+
+def weighted_pair(S=3, nviews=10, weights=None):
+    g, angles, vol = tiny_problem(nviews=nviews)
+    b = O.fp(g, angles, None, vol)
+    off = np.zeros((nviews, 2))
+    op = O.CBSubsetOperator(S)
+    pb = op.setup(g, angles, off, b, O.Binning.default_3d(nviews))
+    ow = O.CBSubsetWeightOperator(S)
+    w = np.ones_like(b) if weights is None else weights(angles, b)
+    pbw = ow.setup(g, angles, off, b, O.Binning.default_3d(nviews), w)
+    return g, angles, vol, b, op, pb, ow, pbw
+
+
+def test_weighted_subset_op_unit_weights_is_unweighted():
+    g, angles, vol, b, op, pb, ow, pbw = weighted_pair()
+    assert np.array_equal(pb, pbw)
+    y0, y1 = np.zeros(pb.shape), np.zeros(pb.shape)
+    op.mult_M(vol, y0)
+    ow.mult_M(vol, y1)
+    assert np.array_equal(y0, y1)
+    x0, x1 = np.zeros(g.vol_shape), np.zeros(g.vol_shape)
+    op.mult_MH(pb, x0)
+    ow.mult_MH(pb, x1)
+    assert np.array_equal(x0, x1)
+
+
+def test_weighted_subset_op_weights_follow_their_views():
+    """Weights 1 + angle/360, a function of each view's own angle: every
+    weighted subset projection is the unweighted one times 1 + a/360 for the
+    angle the subset's projector holds for that view, so the weights are
+    permuted with the views (a transposed permutation fails)."""
+    wfun = lambda a, b: np.broadcast_to((1.0 + a.astype(np.float64) / 360.0)[:, None, None], b.shape).copy()
+    g, angles, vol, b, op, pb, ow, pbw = weighted_pair(weights=wfun)
+    assert np.allclose(pbw, pb * (1.0 + op.angles.astype(np.float64) / 360.0)[:, None, None], rtol=1e-12)
+    for s in range(3):
+        nv = op.codomain[s][0]
+        y0, y1 = np.zeros((nv, g.nv, g.nu)), np.zeros((nv, g.nv, g.nu))
+        op.mult_M_subset(vol, y0, s)
+        ow.mult_M_subset(vol, y1, s)
+        sub_angles = op.ops[s].all_angles.astype(np.float64)
+        assert np.allclose(y1, y0 * (1.0 + sub_angles / 360.0)[:, None, None], rtol=1e-12, atol=0)
+        # accumulate adds the weighted projection
+        y2 = y1.copy()
+        ow.mult_M_subset(vol, y2, s, accumulate=True)
+        assert np.allclose(y2, 2.0 * y1, rtol=1e-12)
+
+
+def test_weighted_subset_op_zero_weights_drop_views():
+    """Views weighted zero take no part in the backprojection."""
+    drop = {2, 7}
+    wfun = lambda a, b: np.stack([np.zeros(b.shape[1:]) if k in drop else np.ones(b.shape[1:])
+                                  for k in range(b.shape[0])])
+    g, angles, vol, b, op, pb, ow, pbw = weighted_pair(weights=wfun)
+    y = np.random.default_rng(2).random(pb.shape)
+    yz = y.copy()
+    for i, p in enumerate(op.perm):
+        if p in drop:
+            yz[i] = 0.0
+    x0, x1 = np.zeros(g.vol_shape), np.zeros(g.vol_shape)
+    op.mult_MH(yz, x0)
+    ow.mult_MH(y, x1)
+    assert np.allclose(x0, x1, rtol=1e-12, atol=1e-15)
+
+
+def test_weighted_solvers_uniform_weights_cancel():
+    """A uniform weight c scales data, M and M^H alike; OS-SART's row/column
+    normalisers and osMOMSolverD's preconditioner cancel it (no TV)."""
+    wfun = lambda a, b: np.full(b.shape, 2.5)
+    g, angles, vol, b, op, pb, ow, pbw = weighted_pair(weights=wfun)
+    assert np.allclose(O.os_sart(ow, pbw, 2, beta=0.8), O.os_sart(op, pb, 2, beta=0.8), rtol=1e-9, atol=1e-12)
+    assert np.allclose(O.os_mom(ow, pbw, 2), O.os_mom(op, pb, 2), rtol=1e-9, atol=1e-12)
