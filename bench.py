@@ -295,7 +295,7 @@ def main():
     kms = fp_ms if kind == "fp" else bp_ms
     ab = algorithmic_bytes(kind, nvox, nviews, nrays, nsamp, s_vol, s_vol)
     achieved = ab / (kms / 1e3) / 1e9
-    kname = "k_fp_col" if kind == "fp" else "k_bp"
+    kname = ctx.fp_kernel(g, 0, nviews, args.storage) if kind == "fp" else "k_bp"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": read_traffic(kname, "%s_%s" % (cfg.name, args.storage)),

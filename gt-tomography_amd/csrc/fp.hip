@@ -33,6 +33,9 @@
 //
 // Engine 0: one thread per ray, 16x16 detector tiles, x-fastest padded
 // volume, 8 gathers per sample.  Also the basis of the exact transpose.
+#include <algorithm>
+#include <cmath>
+
 #include "cbct_internal.h"
 
 namespace {
@@ -235,6 +238,28 @@ __device__ __forceinline__ float2 bload2<unsigned short>(__amdgpu_buffer_rsrc_t 
   return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
 }
 
+// Two consecutive z as kept in registers by the cached column variant.
+template <typename T>
+struct PairWord;
+template <>
+struct PairWord<float> {
+  using type = float2;
+  __device__ static float2 load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return bload2<float>(r, voff, soff);
+  }
+  __device__ static float2 unpack(float2 w) { return w; }
+};
+template <>
+struct PairWord<unsigned short> {
+  using type = unsigned;
+  __device__ static unsigned load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return (unsigned)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+  }
+  __device__ static float2 unpack(unsigned w) {
+    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+  }
+};
+
 // Order one wave's LDS writes before its later LDS reads of other lanes'
 // data (and the reads before the next writes).  A wave's LDS operations
 // complete in issue order, so only the compiler has to be held back.
@@ -251,12 +276,22 @@ __device__ __forceinline__ void wave_lds_sync() {
 //     z clamp (a column-level range check instead), two consecutive z per
 //     lane and load (BF16 storage: one dword per lane instead of two 16-bit
 //     loads, 1.5x faster there).
-// Both produce the same bits as the per-ray engine.
-template <typename T, int RPL, bool SSTEP, int ZP>
-__global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restrict__ views,
-                                               const T *__restrict__ vt, float *__restrict__ proj,
-                                               int accumulate, unsigned ngroups, unsigned nblocks,
-                                               unsigned vt_bytes) {
+//   NCH > 0 (with SSTEP, ZP = 2): the z window of the whole ray (all its
+//     steps, all shared rows) is fixed once; when it spans at most NCH
+//     chunks of 128 z, the four corner z-lines of the current (x, y) cell
+//     stay in registers, and a step into a neighbouring cell keeps the lines
+//     the two cells share (two of four for a move along x or y, one for a
+//     diagonal move) and loads only the others.  The column's (x, y) is
+//     wave-uniform, so the case split is a scalar branch.
+//     The lines are kept as (00, 10, 01, 11) of the cell and shifted by
+//     register moves.  Every line load fetches all NCH chunks (no per-chunk
+//     branches); chunks past the window are never read.
+// All produce the same bits as the per-ray engine.
+template <typename T, int RPL, bool SSTEP, int ZP, int NCH>
+__device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__restrict__ views,
+                                            const T *__restrict__ vt, float *__restrict__ proj,
+                                            int accumulate, unsigned ngroups, unsigned nblocks,
+                                            unsigned vt_bytes) {
   extern __shared__ float zl[];  // one interpolated z-line (whole 128-wide chunks)
   const unsigned W = xcd_remap(blockIdx.x, nblocks);
   const int i = (int)(W % (unsigned)P.nu);
@@ -337,7 +372,107 @@ __global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restri
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *)vt, (short)0, (int)vt_bytes, 0x00020000);
     const unsigned lane_off = (unsigned)lane * esz;
-    for (int m = 0; m < P.n; ++m) {
+    bool done = false;
+    if constexpr (NCH > 0) {
+      static_assert(ZP == 2, "cached z-lines use pair loads");
+      // The window: per step the rays read z in [zlo(m), zhi(m)], both linear
+      // in m through the same fmaf as below (so monotone), hence extreme at
+      // the first or last step.
+      const float ml = (float)(P.n - 1);
+      const int zlo_min = __builtin_amdgcn_readfirstlane((int)fminf(pzl, fmaf(ml, qzl, pzl)));
+      const int zhi_max = __builtin_amdgcn_readfirstlane((int)fmaxf(pzh, fmaf(ml, qzh, pzh))) + 1;
+      const int zb0 = zlo_min & ~1;
+      const int nchw = (zhi_max - zb0 + 128) >> 7;
+      if (nchw <= NCH) {
+        done = true;
+        // the lines as loaded: FP32 pairs, or BF16 pairs packed in one word
+        // (unpacked in the combine, which keeps the registers, and with them
+        // the occupancy, of the uncached variant)
+        using W = typename PairWord<T>::type;
+        W L00[NCH], L10[NCH], L01[NCH], L11[NCH];
+        const unsigned zoff = (unsigned)zb0 * esz;
+        int cx = -8, cy = -8;   // no cell loaded yet
+        auto load_line = [&](W(&L)[NCH], unsigned B) {
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) L[c] = PairWord<T>::load(rsrc, 2 * lane_off + (unsigned)(128 * c) * esz, B);
+        };
+        auto copy_line = [&](W(&D)[NCH], const W(&S)[NCH]) {
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) D[c] = S[c];
+        };
+        const float *zb = zl - zb0;
+        for (int m = 0; m < P.n; ++m) {
+          const float fm = (float)m;
+          const float sx = __builtin_amdgcn_fmed3f(fmaf(fm, qx, px0), 0.0f, P.cx);
+          const float sy = __builtin_amdgcn_fmed3f(fmaf(fm, qy, py0), 0.0f, P.cy);
+          const float ax = __builtin_amdgcn_fractf(sx), ay = __builtin_amdgcn_fractf(sy);
+          const int ix = __builtin_amdgcn_readfirstlane((int)sx);
+          const int iy = __builtin_amdgcn_readfirstlane((int)sy);
+          if (ix != cx || iy != cy) {
+            const int dx = ix - cx, dy = iy - cy;
+            const unsigned B = ((unsigned)iy * P.px + (unsigned)ix) * PZ * esz + zoff;
+            if (dy == 0 && dx == 1) {
+              copy_line(L00, L10); copy_line(L01, L11);
+              load_line(L10, B + so_x); load_line(L11, B + so_xy);
+            } else if (dy == 0 && dx == -1) {
+              copy_line(L10, L00); copy_line(L11, L01);
+              load_line(L00, B); load_line(L01, B + so_y);
+            } else if (dx == 0 && dy == 1) {
+              copy_line(L00, L01); copy_line(L10, L11);
+              load_line(L01, B + so_y); load_line(L11, B + so_xy);
+            } else if (dx == 0 && dy == -1) {
+              copy_line(L01, L00); copy_line(L11, L10);
+              load_line(L00, B); load_line(L10, B + so_x);
+            } else if (dx == 1 && dy == 1) {
+              copy_line(L00, L11);
+              load_line(L10, B + so_x); load_line(L01, B + so_y); load_line(L11, B + so_xy);
+            } else if (dx == 1 && dy == -1) {
+              copy_line(L01, L10);
+              load_line(L00, B); load_line(L10, B + so_x); load_line(L11, B + so_xy);
+            } else if (dx == -1 && dy == 1) {
+              copy_line(L10, L01);
+              load_line(L00, B); load_line(L01, B + so_y); load_line(L11, B + so_xy);
+            } else if (dx == -1 && dy == -1) {
+              copy_line(L11, L00);
+              load_line(L00, B); load_line(L10, B + so_x); load_line(L01, B + so_y);
+            } else {
+              load_line(L00, B); load_line(L10, B + so_x);
+              load_line(L01, B + so_y); load_line(L11, B + so_xy);
+            }
+            cx = ix;
+            cy = iy;
+          }
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) {
+            const float2 v00 = PairWord<T>::unpack(L00[c]), v10 = PairWord<T>::unpack(L10[c]);
+            const float2 v01 = PairWord<T>::unpack(L01[c]), v11 = PairWord<T>::unpack(L11[c]);
+            float2 o;
+            {
+              const float c0 = fmaf(ax, v10.x - v00.x, v00.x);
+              const float c1 = fmaf(ax, v11.x - v01.x, v01.x);
+              o.x = fmaf(ay, c1 - c0, c0);
+            }
+            {
+              const float c0 = fmaf(ax, v10.y - v00.y, v00.y);
+              const float c1 = fmaf(ax, v11.y - v01.y, v01.y);
+              o.y = fmaf(ay, c1 - c0, c0);
+            }
+            *reinterpret_cast<float2 *>(&zl[2 * lane + 128 * c]) = o;
+          }
+          wave_lds_sync();
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) {
+            const float tz = fmaf(fm, qz[r], pz[r]);  // in [0, nz+1]: checked above
+            const int iz = (int)tz;
+            const float az = __builtin_amdgcn_fractf(tz);
+            const float a = zb[iz], b = zb[iz + 1];
+            acc[r] += fmaf(az, b - a, a);
+          }
+          wave_lds_sync();
+        }
+      }
+    }
+    for (int m = 0; !done && m < P.n; ++m) {
       // Uniform per-step values: the float part on the VALU (same
       // expressions as the per-ray loop), the integer part moved to scalar
       // registers so the addressing below is scalar.
@@ -495,6 +630,24 @@ __global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restri
   }
 }
 
+template <typename T, int RPL, bool SSTEP, int ZP>
+__global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restrict__ views,
+                                               const T *__restrict__ vt, float *__restrict__ proj,
+                                               int accumulate, unsigned ngroups, unsigned nblocks,
+                                               unsigned vt_bytes) {
+  fp_col_body<T, RPL, SSTEP, ZP, 0>(P, views, vt, proj, accumulate, ngroups, nblocks, vt_bytes);
+}
+
+// The cached variant, held to five waves per SIMD (it is latency-bound: five
+// waves measured faster than four with more registers).
+template <typename T, int RPL, int NCH>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+k_fp_colc(FPParams P, const ViewD *__restrict__ views, const T *__restrict__ vt,
+          float *__restrict__ proj, int accumulate, unsigned ngroups, unsigned nblocks,
+          unsigned vt_bytes) {
+  fp_col_body<T, RPL, true, 2, NCH>(P, views, vt, proj, accumulate, ngroups, nblocks, vt_bytes);
+}
+
 // ------------------------------------------------------------ layouts
 // Element conversion of the padded copies: identity, or FP32 -> BF16 when the
 // context keeps its working copies in BF16.
@@ -619,25 +772,42 @@ int make_params(cbct_ctx ctx, const cbct_geom *g, float spp, FPParams &P) {
   return CBCT_OK;
 }
 
-template <typename T, int RPL, bool SSTEP, int ZP>
+template <typename T, int RPL, bool SSTEP, int ZP, int NCH>
 void launch_col(const FPParams &P, hipStream_t s, const ViewD *views, const T *vt, float *proj,
                 int acc, int nviews, unsigned vt_bytes) {
   const unsigned ngroups = (unsigned)((P.nv + 64 * RPL - 1) / (64 * RPL));
   const unsigned nblocks = (unsigned)P.nu * ngroups * (unsigned)nviews;
-  const size_t lds = ((P.pz + 127) / 128) * 128 * sizeof(float);
-  hipLaunchKernelGGL((k_fp_col<T, RPL, SSTEP, ZP>), dim3(nblocks), dim3(64), lds, s, P, views, vt,
-                     proj, acc, ngroups, nblocks, vt_bytes);
+  // one z-line; the cached variants write all NCH chunks of their window
+  const size_t zline = ((P.pz + 127) / 128) * 128;
+  const size_t lds = (zline > (size_t)128 * NCH ? zline : (size_t)128 * NCH) * sizeof(float);
+  if constexpr (NCH > 0)
+    hipLaunchKernelGGL((k_fp_colc<T, RPL, NCH>), dim3(nblocks), dim3(64), lds, s, P, views, vt, proj,
+                       acc, ngroups, nblocks, vt_bytes);
+  else
+    hipLaunchKernelGGL((k_fp_col<T, RPL, SSTEP, ZP>), dim3(nblocks), dim3(64), lds, s, P, views, vt,
+                       proj, acc, ngroups, nblocks, vt_bytes);
 }
 
 // variant 1: vector step setup, one z per load; variant 2: scalar step
-// setup, two z per load
+// setup, two z per load; variants 3 and 4: variant 2 with the corner z-lines
+// cached in registers over a whole-ray window of at most 2 (3: 3) chunks of
+// 128 z (k_fp_colc).
 template <typename T, int RPL>
 void launch_col_w(const FPParams &P, int variant, hipStream_t s, const ViewD *views, const T *vt,
                   float *proj, int acc, int nviews, unsigned vt_bytes) {
-  if (variant == 2)
-    launch_col<T, RPL, true, 2>(P, s, views, vt, proj, acc, nviews, vt_bytes);
-  else
-    launch_col<T, RPL, false, 1>(P, s, views, vt, proj, acc, nviews, vt_bytes);
+  switch (variant) {
+    case 2: launch_col<T, RPL, true, 2, 0>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 3: launch_col<T, RPL, true, 2, 2>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    case 4: launch_col<T, RPL, true, 2, 3>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+    default: launch_col<T, RPL, false, 1, 0>(P, s, views, vt, proj, acc, nviews, vt_bytes); break;
+  }
+}
+
+// Rows per lane that dispatch_col picks for max_rpl.
+inline int rows_per_lane(int nv, int max_rpl) {
+  const int need = (nv + 63) / 64;
+  const int groups = (need + max_rpl - 1) / max_rpl;
+  return (need + groups - 1) / groups;
 }
 
 // Rays per lane: whole 64-row blocks, at most `max_rpl` per lane (1..12;
@@ -646,9 +816,7 @@ void launch_col_w(const FPParams &P, int variant, hipStream_t s, const ViewD *vi
 template <typename T>
 void dispatch_col(const FPParams &P, int variant, int max_rpl, hipStream_t s, const ViewD *views,
                   const T *vt, float *proj, int acc, int nviews, unsigned vt_bytes) {
-  const int need = (P.nv + 63) / 64;
-  const int groups = (need + max_rpl - 1) / max_rpl;
-  const int rpl = (need + groups - 1) / groups;
+  const int rpl = rows_per_lane(P.nv, max_rpl);
   switch (rpl) {
     case 1: launch_col_w<T, 1>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
     case 2: launch_col_w<T, 2>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
@@ -665,22 +833,63 @@ void dispatch_col(const FPParams &P, int variant, int max_rpl, hipStream_t s, co
   }
 }
 
+// Engine 1's choice.  The cached variant pays when every row group's
+// whole-ray z window fits its two chunks.  A bound: a ray through detector
+// height v (offset included) is at z = v t / SDD where t, its depth along the
+// central ray, lies in [SAD - R, SAD + R] inside the box (R: half the box's
+// xy diagonal); the window spans the extremes of v t over the group's rows
+// and that depth range, plus the texel, the alignment and rounding slack.
+// Otherwise the per-step variant measured fastest for the storage type.
+int default_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
+                    int per_step) {
+  const int rpl = rows_per_lane(P.nv, ctx->fp_max_rpl);
+  float omin = 0.0f, omax = 0.0f;
+  for (int k = 0; k < nviews; ++k) {
+    const float ov = ctx->offsets[2 * (size_t)(view0 + k) + 1];
+    omin = k ? fminf(omin, ov) : ov;
+    omax = k ? fmaxf(omax, ov) : ov;
+  }
+  const double R = 0.5 * std::sqrt((double)g->lx * g->lx + (double)g->ly * g->ly);
+  const double t0 = (double)g->sad - R, t1 = (double)g->sad + R;
+  const double pitch = (double)g->fov_v / g->nv, vz = (double)g->lz / g->nz;
+  double worst = 0.0;
+  for (int row0 = 0; row0 < P.nv; row0 += 64 * rpl) {
+    const int row1 = std::min(P.nv, row0 + 64 * rpl) - 1;
+    const double v0 = ((row0 + 0.5) / P.nv - 0.5) * g->fov_v + omin - 0.5 * pitch;
+    const double v1 = ((row1 + 0.5) / P.nv - 0.5) * g->fov_v + omax + 0.5 * pitch;
+    const double e[4] = {v0 * t0, v0 * t1, v1 * t0, v1 * t1};
+    const double lo = std::min(std::min(e[0], e[1]), std::min(e[2], e[3]));
+    const double hi = std::max(std::max(e[0], e[1]), std::max(e[2], e[3]));
+    worst = std::max(worst, (hi - lo) / g->sdd / vz);
+  }
+  return worst + 4.0 <= 256.0 ? 3 : per_step;
+}
+
+// The variant cbct_fp runs: 0 = per ray (engine 0, or a volume or z-line
+// beyond the column engine's 32-bit byte offsets / 64 KB of LDS); engine 1:
+// default_variant; engine 2: the other per-step variant; engines 3 and 4: the
+// cached variants.
+int select_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
+                   size_t esz) {
+  const size_t vt_bytes = (size_t)P.pzx * padded_dim(g->ny) * esz;
+  const size_t zline_bytes = ((P.pz + 127) / 128) * 128 * sizeof(float);
+  if (ctx->fp_engine == 0 || vt_bytes + (size_t)P.pzx * esz * 2 >= (1ull << 31) || zline_bytes > 65536)
+    return 0;
+  const int per_step = esz == 2 ? 2 : 1;
+  if (ctx->fp_engine == 2) return 3 - per_step;
+  if (ctx->fp_engine >= 3) return ctx->fp_engine;
+  return default_variant(ctx, g, P, view0, nviews, per_step);
+}
+
 // TI: the caller's volume type; T: the padded working copy's type.
 template <typename TI, typename T>
 int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
            const TI *vol, float *proj, int accumulate, hipStream_t s) {
   const size_t esz = sizeof(T);
   const ViewD *views = ctx->d_views_d + view0;
-  // the column engine addresses the z-fastest copy with 32-bit byte offsets
   const size_t vt_bytes = (size_t)P.pzx * padded_dim(g->ny) * esz;
-  // LDS: one z-line per workgroup
-  const size_t zline_bytes = ((P.pz + 127) / 128) * 128 * sizeof(float);
-  // engine 1: the variant measured fastest for the storage type; engine 2:
-  // the other one
-  const int best = sizeof(T) == 2 ? 2 : 1;
-  const int variant = ctx->fp_engine == 2 ? 3 - best : best;
-  if (ctx->fp_engine == 0 || vt_bytes + (size_t)P.pzx * esz * 2 >= (1ull << 31) ||
-      zline_bytes > 65536) {
+  const int variant = select_variant(ctx, g, P, view0, nviews, esz);
+  if (variant == 0) {
     const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
     int rc = cbct_reserve(ctx, ctx->vol_pad, npad * esz);
     if (rc) return rc;
@@ -721,7 +930,7 @@ extern "C" {
 
 int cbct_fp_engine(cbct_ctx ctx, int engine) {
   if (!ctx) return CBCT_ERR_ARG;
-  if (engine < 0 || engine > 2)
+  if (engine < 0 || engine > 4)
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_fp_engine: engine %d not available", engine);
   ctx->fp_engine = engine;
   return CBCT_OK;
@@ -733,6 +942,18 @@ int cbct_fp_tile(cbct_ctx ctx, int max_rows_per_lane) {
     return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_fp_tile: rows per lane %d outside 1..12", max_rows_per_lane);
   ctx->fp_max_rpl = max_rows_per_lane;
   return CBCT_OK;
+}
+
+const char *cbct_fp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, int dtype,
+                           float spp) {
+  if (!ctx || !g || cbct_check_geom(ctx, g, "cbct_fp_kernel") ||
+      cbct_check_views(ctx, view0, nviews, "cbct_fp_kernel"))
+    return "";
+  FPParams P;
+  if (make_params(ctx, g, spp, P)) return "";
+  const size_t esz = (dtype == CBCT_BF16 || ctx->work_dtype == CBCT_BF16) ? 2 : 4;
+  const int v = select_variant(ctx, g, P, view0, nviews, esz);
+  return v == 0 ? "k_fp_ray" : v >= 3 ? "k_fp_colc" : "k_fp_col";
 }
 
 int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol, int dtype,

@@ -102,6 +102,7 @@ def load():
         "cbct_bp": (I, [P, G, I, I, P, I, P, I, I, P]),
         "cbct_fp_transpose": (I, [P, G, I, I, P, P, F, I, P]),
         "cbct_fp_engine": (I, [P, I]),
+        "cbct_fp_kernel": (ctypes.c_char_p, [P, G, I, I, I, F]),
         "cbct_fp_tile": (I, [P, I]),
         "cbct_bp_engine": (I, [P, I]),
         "cbct_storage": (I, [P, I]),
@@ -246,6 +247,13 @@ class Context:
                                      _dtype_code(vol), _ptr(proj), samples_per_pixel,
                                      int(accumulate), _stream(stream)))
         return proj
+
+    def fp_kernel(self, g: Geom, view0=0, nviews=None, dtype="f32", samples_per_pixel=1.5):
+        """Name of the kernel fp() would launch (for profiler output)."""
+        n = self.nviews() - view0 if nviews is None else nviews
+        code = CBCT_BF16 if dtype in ("bf16", "bfloat16") else CBCT_F32
+        return self.lib.cbct_fp_kernel(self.h, ctypes.byref(g), view0, n, code,
+                                       samples_per_pixel).decode()
 
     def fp_engine(self, engine):
         """0 = one thread per ray, >= 1 = detector-column sharing variants (1 default)."""

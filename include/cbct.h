@@ -97,12 +97,21 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
             int proj_dtype, float *vol, int fbp, int accumulate, void *stream);
 
 /* Forward-projector engine: 1 = detector-column sharing (default; one wave per
- * detector column interpolates the shared z-line once per step, in the
- * variant measured fastest for the volume's storage type), 2 = the other
- * column-sharing variant, 0 = one thread per ray.  All give bitwise-identical
- * projections.  No reference counterpart (the reference has one
- * texture-based kernel). */
+ * detector column interpolates the shared z-line once per step: the variant
+ * that keeps the column's four corner z-lines in registers across steps when
+ * every row group's z window fits two chunks of 128 z, else the per-step
+ * variant measured fastest for the storage type), 2 = the other per-step
+ * variant, 3 / 4 = the cached variant with a two / three chunk window (falls
+ * back to per-step loads per column when the window is larger), 0 = one
+ * thread per ray.  All give bitwise-identical projections.  No reference
+ * counterpart (the reference has one texture-based kernel). */
 int cbct_fp_engine(cbct_ctx ctx, int engine);
+
+/* Name of the kernel cbct_fp would launch for these arguments under the
+ * context's current settings ("k_fp_ray", "k_fp_col" or "k_fp_colc"; "" on
+ * bad arguments), for matching profiler output.  No reference counterpart. */
+const char *cbct_fp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews,
+                           int vol_dtype, float samples_per_pixel);
 
 /* Column-engine tile: the most detector rows one lane carries (1..12,
  * default 8).  Columns with more rows than 64 x this are split into row

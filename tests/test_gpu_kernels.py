@@ -77,15 +77,23 @@ def test_fp_parity(ctx, name):
 
 
 @pytest.mark.parametrize("dtype", ["f32", "bf16"])
-@pytest.mark.parametrize("name", list(CASES) + ["tall", "wide"])
+@pytest.mark.parametrize("name", list(CASES) + ["tall", "wide", "deep"])
 def test_fp_engines_bitwise_identical(ctx, name, dtype):
-    """The detector-column engine and the per-ray engine give identical bits.
-    'tall' has most rays leaving through the top/bottom faces (per-ray
-    fallback); 'wide' has detector columns that miss the box entirely."""
+    """The detector-column engines (1-4) and the per-ray engine give
+    identical bits.  'tall' has most rays leaving through the top/bottom
+    faces (per-ray fallback); 'wide' has detector columns that miss the box
+    entirely; 'deep' has a 600-voxel z range: with 12 rows per lane one
+    row group's z window spans 5 chunks of 128 z, more than the cached
+    engines hold (they fall back to per-step loads), with the default tile
+    it spans 3."""
+    tiles = (8,)
     if name == "tall":
         g, angles, offsets = geom(20, 20, 5, 30, 150, fov=(40.0, 60.0)), angles_full(7), None
     elif name == "wide":
         g, angles, offsets = geom(12, 12, 12, 40, 70, fov=(400.0, 30.0)), angles_full(5), None
+    elif name == "deep":
+        g, angles, offsets = geom(8, 8, 600, 6, 720, fov=(20.0, 900.0)), angles_full(3), None
+        tiles = (8, 12)
     else:
         g, angles, offsets = make_case(name)
     print_case("engines", name, dtype, g)
@@ -93,14 +101,17 @@ def test_fp_engines_bitwise_identical(ctx, name, dtype):
     tdt = torch.float32 if dtype == "f32" else torch.bfloat16
     ctx.set_views(angles, offsets)
     outs = []
-    for e in (0, 1, 2):   # per ray; the column-sharing variants
-        ctx.fp_engine(e)
-        out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
-        ctx.fp(g, dev(vol, tdt), out)
-        torch.cuda.synchronize()
-        outs.append(out)
+    for tile in tiles:
+        ctx.fp_tile(tile)
+        for e in (0, 1, 2, 3, 4):   # per ray; the column-sharing variants
+            ctx.fp_engine(e)
+            out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
+            ctx.fp(g, dev(vol, tdt), out)
+            torch.cuda.synchronize()
+            outs.append(out)
     ctx.fp_engine(1)
-    for e in (1, 2):
+    ctx.fp_tile(8)
+    for e in range(1, len(outs)):
         assert torch.equal(outs[0], outs[e]), e
     if dtype == "f32":
         assert rel_l2(host(outs[1]), O.fp(to_oracle(g), angles, offsets, vol)) <= TOL
