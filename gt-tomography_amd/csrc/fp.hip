@@ -810,13 +810,12 @@ inline int rows_per_lane(int nv, int max_rpl) {
   return (need + groups - 1) / groups;
 }
 
-// Rays per lane: whole 64-row blocks, at most `max_rpl` per lane (1..12;
-// rows are split into groups beyond that).  More rows per wave share one
+// Launch with `rpl` rays per lane (1..12; rows_per_lane splits the rows
+// into equal groups of whole 64-row blocks).  More rows per wave share one
 // z-line load; fewer keep registers, and so occupancy, in check.
 template <typename T>
-void dispatch_col(const FPParams &P, int variant, int max_rpl, hipStream_t s, const ViewD *views,
+void dispatch_col(const FPParams &P, int variant, int rpl, hipStream_t s, const ViewD *views,
                   const T *vt, float *proj, int acc, int nviews, unsigned vt_bytes) {
-  const int rpl = rows_per_lane(P.nv, max_rpl);
   switch (rpl) {
     case 1: launch_col_w<T, 1>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
     case 2: launch_col_w<T, 2>(P, variant, s, views, vt, proj, acc, nviews, vt_bytes); break;
@@ -839,45 +838,60 @@ void dispatch_col(const FPParams &P, int variant, int max_rpl, hipStream_t s, co
 // central ray, lies in [SAD - R, SAD + R] inside the box (R: half the box's
 // xy diagonal); the window spans the extremes of v t over the group's rows
 // and that depth range, plus the texel, the alignment and rounding slack.
-// Otherwise the per-step variant measured fastest for the storage type.
-int default_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
-                    int per_step) {
-  const int rpl = rows_per_lane(P.nv, ctx->fp_max_rpl);
+// The largest tile (rows per lane, at least 2) whose groups fit is taken;
+// if none does, the per-step variant measured fastest for the storage type
+// with the configured tile.
+struct FPChoice {
+  int variant;
+  int rpl;
+};
+
+double window_bound(const cbct_geom *g, int nv, int rpl, float omin, float omax) {
+  const double R = 0.5 * std::sqrt((double)g->lx * g->lx + (double)g->ly * g->ly);
+  const double t0 = (double)g->sad - R, t1 = (double)g->sad + R;
+  const double pitch = (double)g->fov_v / nv, vz = (double)g->lz / g->nz;
+  double worst = 0.0;
+  for (int row0 = 0; row0 < nv; row0 += 64 * rpl) {
+    const int row1 = std::min(nv, row0 + 64 * rpl) - 1;
+    const double v0 = ((row0 + 0.5) / nv - 0.5) * g->fov_v + omin - 0.5 * pitch;
+    const double v1 = ((row1 + 0.5) / nv - 0.5) * g->fov_v + omax + 0.5 * pitch;
+    const double e[4] = {v0 * t0, v0 * t1, v1 * t0, v1 * t1};
+    const double lo = std::min(std::min(e[0], e[1]), std::min(e[2], e[3]));
+    const double hi = std::max(std::max(e[0], e[1]), std::max(e[2], e[3]));
+    worst = std::max(worst, (hi - lo) / g->sdd / vz);
+  }
+  return worst + 4.0;
+}
+
+FPChoice default_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
+                         int per_step) {
   float omin = 0.0f, omax = 0.0f;
   for (int k = 0; k < nviews; ++k) {
     const float ov = ctx->offsets[2 * (size_t)(view0 + k) + 1];
     omin = k ? fminf(omin, ov) : ov;
     omax = k ? fmaxf(omax, ov) : ov;
   }
-  const double R = 0.5 * std::sqrt((double)g->lx * g->lx + (double)g->ly * g->ly);
-  const double t0 = (double)g->sad - R, t1 = (double)g->sad + R;
-  const double pitch = (double)g->fov_v / g->nv, vz = (double)g->lz / g->nz;
-  double worst = 0.0;
-  for (int row0 = 0; row0 < P.nv; row0 += 64 * rpl) {
-    const int row1 = std::min(P.nv, row0 + 64 * rpl) - 1;
-    const double v0 = ((row0 + 0.5) / P.nv - 0.5) * g->fov_v + omin - 0.5 * pitch;
-    const double v1 = ((row1 + 0.5) / P.nv - 0.5) * g->fov_v + omax + 0.5 * pitch;
-    const double e[4] = {v0 * t0, v0 * t1, v1 * t0, v1 * t1};
-    const double lo = std::min(std::min(e[0], e[1]), std::min(e[2], e[3]));
-    const double hi = std::max(std::max(e[0], e[1]), std::max(e[2], e[3]));
-    worst = std::max(worst, (hi - lo) / g->sdd / vz);
+  for (int r = ctx->fp_max_rpl; r >= 2; --r) {
+    const int rpl = rows_per_lane(P.nv, r);
+    if (rpl >= 2 && window_bound(g, P.nv, rpl, omin, omax) <= 256.0) return {3, rpl};
   }
-  return worst + 4.0 <= 256.0 ? 3 : per_step;
+  return {per_step, rows_per_lane(P.nv, ctx->fp_max_rpl)};
 }
 
-// The variant cbct_fp runs: 0 = per ray (engine 0, or a volume or z-line
-// beyond the column engine's 32-bit byte offsets / 64 KB of LDS); engine 1:
-// default_variant; engine 2: the other per-step variant; engines 3 and 4: the
-// cached variants.
-int select_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
-                   size_t esz) {
+// The variant cbct_fp runs and its tile: variant 0 = per ray (engine 0, or a
+// volume or z-line beyond the column engine's 32-bit byte offsets / 64 KB of
+// LDS); engine 1: default_variant; engine 2: the other per-step variant;
+// engines 3 and 4: the cached variants (configured tile).
+FPChoice select_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
+                        size_t esz) {
   const size_t vt_bytes = (size_t)P.pzx * padded_dim(g->ny) * esz;
   const size_t zline_bytes = ((P.pz + 127) / 128) * 128 * sizeof(float);
+  const int rpl = rows_per_lane(P.nv, ctx->fp_max_rpl);
   if (ctx->fp_engine == 0 || vt_bytes + (size_t)P.pzx * esz * 2 >= (1ull << 31) || zline_bytes > 65536)
-    return 0;
+    return {0, rpl};
   const int per_step = esz == 2 ? 2 : 1;
-  if (ctx->fp_engine == 2) return 3 - per_step;
-  if (ctx->fp_engine >= 3) return ctx->fp_engine;
+  if (ctx->fp_engine == 2) return {3 - per_step, rpl};
+  if (ctx->fp_engine >= 3) return {ctx->fp_engine, rpl};
   return default_variant(ctx, g, P, view0, nviews, per_step);
 }
 
@@ -888,8 +902,8 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
   const size_t esz = sizeof(T);
   const ViewD *views = ctx->d_views_d + view0;
   const size_t vt_bytes = (size_t)P.pzx * padded_dim(g->ny) * esz;
-  const int variant = select_variant(ctx, g, P, view0, nviews, esz);
-  if (variant == 0) {
+  const FPChoice choice = select_variant(ctx, g, P, view0, nviews, esz);
+  if (choice.variant == 0) {
     const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
     int rc = cbct_reserve(ctx, ctx->vol_pad, npad * esz);
     if (rc) return rc;
@@ -917,7 +931,7 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
                      P.px, P.pz);
   CBCT_LAUNCH_CHECK(ctx, "k_pad_transpose");
   cbct_prof_mark(ctx, PROF_FP, 0, s);
-  dispatch_col<T>(P, variant, ctx->fp_max_rpl, s, views, vt, proj, accumulate, nviews,
+  dispatch_col<T>(P, choice.variant, choice.rpl, s, views, vt, proj, accumulate, nviews,
                   (unsigned)vt_bytes);
   cbct_prof_mark(ctx, PROF_FP, 1, s);
   CBCT_LAUNCH_CHECK(ctx, "k_fp_col");
@@ -952,7 +966,7 @@ const char *cbct_fp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nvie
   FPParams P;
   if (make_params(ctx, g, spp, P)) return "";
   const size_t esz = (dtype == CBCT_BF16 || ctx->work_dtype == CBCT_BF16) ? 2 : 4;
-  const int v = select_variant(ctx, g, P, view0, nviews, esz);
+  const int v = select_variant(ctx, g, P, view0, nviews, esz).variant;
   return v == 0 ? "k_fp_ray" : v >= 3 ? "k_fp_colc" : "k_fp_col";
 }
 

@@ -27,23 +27,29 @@ namespace {
 
 thread_local std::string g_err;
 
+// Runs an entry point; device blocks the call cached for reuse are freed on
+// the way out.
 template <class F>
 int guarded(F &&f) {
+  int rc = 0;
   try {
     f();
-    return 0;
   } catch (const std::exception &e) {
     g_err = e.what();
-    return -1;
+    rc = -1;
   }
+  trim_device_cache();
+  return rc;
 }
 
 // CBSubsetOperator, or CBSubsetWeightOperator when weights (host, the
 // projections' shape) are given, set up on the scan (cuCBOS_reconstruct.cpp:481-503).
+// Unweighted scans may borrow the caller's projections: they stay in
+// acquisition order on the host (see subset_data).
 std::shared_ptr<CBSubsetOperator> make_subset_operator(int nsubsets, const Scan &s, const float *weights) {
   if (!weights) {
     auto E = std::make_shared<CBSubsetOperator>(nsubsets);
-    E->setup(s.acq, s.binning, s.dims_mm);
+    E->setup(s.acq, s.binning, s.dims_mm, false);
     return E;
   }
   auto proj = s.acq->get_projections();
@@ -53,6 +59,14 @@ std::shared_ptr<CBSubsetOperator> make_subset_operator(int nsubsets, const Scan 
   auto E = std::make_shared<CBSubsetWeightOperator>(nsubsets);
   E->setup(s.acq, s.binning, s.dims_mm, w);
   return E;
+}
+
+// The solver's data in subset order on the device: the weighted operator
+// permuted (and weighted) the host array at setup; otherwise the host array
+// is still in acquisition order and is permuted on the device.
+cuNDArray<float> subset_data(const CBSubsetOperator &E, const Scan &s, const float *weights) {
+  if (weights) return cuNDArray<float>(*s.acq->get_projections());
+  return upload_in_subset_order(*s.acq->get_projections(), E.get_permutation());
 }
 
 }  // namespace
@@ -80,7 +94,7 @@ int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles, const float 
                            float *volume_out, double *scale_out) {
   return guarded([&] {
     if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
-    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections, true);
     std::shared_ptr<std::vector<float>> F;
     if (filter) F = std::make_shared<std::vector<float>>(filter, filter + g->nu + 1);
     FDKResult r = FDK_reconstruct_3d(s.acq, s.binning, s.vol_dims, s.dims_mm, F, spp);
@@ -95,7 +109,7 @@ int gtt_fdk_reconstruct_4d(const cbct_geom *g, const float *angles, const float 
                            float *volume_out, float *fdk3d_out, double *scale_out) {
   return guarded([&] {
     if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
-    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections, true);
     std::shared_ptr<std::vector<float>> F;
     if (filter) F = std::make_shared<std::vector<float>>(filter, filter + g->nu + 1);
     const std::vector<size_t> dims3(s.vol_dims.begin(), s.vol_dims.begin() + 3);
@@ -123,10 +137,11 @@ int gtt_os_sart_weighted(const cbct_geom *g, const float *angles, const float *o
   return guarded([&] {
     if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
     if (nsubsets <= 0) throw std::runtime_error("number of subsets must be positive");
-    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections,
+                       weights == nullptr);
     auto E = make_subset_operator(nsubsets, s, weights);
     E->set_domain_dimensions(&s.vol_dims);
-    cuNDArray<float> b(*s.acq->get_projections());
+    cuNDArray<float> b = subset_data(*E, s, weights);
     osSARTSolver<cuNDArray<float>> solver;
     solver.set_encoding_operator(E);
     solver.set_max_iterations(iterations);
@@ -157,10 +172,11 @@ int gtt_os_mom_weighted(const cbct_geom *g, const float *angles, const float *of
   return guarded([&] {
     if (!projections || !volume_out) throw std::runtime_error("illegal array pointer provided");
     if (nsubsets <= 0) throw std::runtime_error("number of subsets must be positive");
-    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections);
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, projections,
+                       weights == nullptr);
     auto E = make_subset_operator(nsubsets, s, weights);
     E->set_domain_dimensions(&s.vol_dims);
-    cuNDArray<float> b(*s.acq->get_projections());
+    cuNDArray<float> b = subset_data(*E, s, weights);
     osMOMSolverD<cuNDArray<float>> solver;
     solver.set_dump(false);
     solver.set_encoding_operator(E);

@@ -264,7 +264,8 @@ std::shared_ptr<std::vector<size_t>> CBSubsetOperator::get_codomain_dimensions(i
 }
 
 void CBSubsetOperator::setup(std::shared_ptr<CBCT_acquisition> acq,
-                             std::shared_ptr<CBCT_binning> binning, floatd3 is_dims_in_mm) {
+                             std::shared_ptr<CBCT_binning> binning, floatd3 is_dims_in_mm,
+                             bool permute_projections) {
   auto geometry = acq->get_geometry();
   std::vector<float> all_angles = geometry->get_angles();
   std::vector<floatd2> all_offsets = geometry->get_offsets();
@@ -302,10 +303,13 @@ void CBSubsetOperator::setup(std::shared_ptr<CBCT_acquisition> acq,
 
   // permute the projection array (:106, :192-212)
   auto proj = acq->get_projections();
-  if (proj && proj->get_number_of_elements()) {
+  if (proj && proj->get_number_of_elements() && !permute_projections) {
+    for (auto p : permutations_)
+      if (p >= proj->get_size(2)) throw std::runtime_error("Error: CBSubsetOperator: illegal index in bin");
+  } else if (proj && proj->get_number_of_elements()) {
     auto dims = proj->dims();
     dims.back() = permutations_.size();
-    auto res = std::make_shared<hoCuNDArray<float>>(dims);
+    auto res = std::make_shared<hoCuNDArray<float>>(dims, no_init);
     const size_t plane = proj->get_size(0) * proj->get_size(1);
     for (size_t i = 0; i < permutations_.size(); ++i) {
       if (permutations_[i] >= proj->get_size(2))
@@ -357,6 +361,25 @@ void CBSubsetOperator::offset_correct(cuNDArray<float> *projections) {
     if (operators_[s]->get_use_offset_correction()) operators_[s]->offset_correct(sets[s].get());
 }
 
+cuNDArray<float> upload_in_subset_order(const hoCuNDArray<float> &projections,
+                                        const std::vector<unsigned int> &permutation) {
+  const size_t nu = projections.get_size(0), nv = projections.get_size(1);
+  const size_t n = projections.get_size(2), plane = nu * nv;
+  for (auto p : permutation)
+    if (p >= n) throw std::runtime_error("Error: CBSubsetOperator: illegal index in bin");
+  cuNDArray<float> raw(projections.dims());
+  copy_h2d(raw.get_data_ptr(), projections.get_data_ptr(), projections.get_number_of_bytes());
+  cuNDArray<float> out(std::vector<size_t>{nu, nv, permutation.size()});
+  for (size_t i = 0; i < permutation.size();) {
+    size_t run = 1;
+    while (i + run < permutation.size() && permutation[i + run] == permutation[i + run - 1] + 1) ++run;
+    copy_d2d(out.get_data_ptr() + plane * i, raw.get_data_ptr() + plane * permutation[i],
+             run * plane * sizeof(float));
+    i += run;
+  }
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 void CBSubsetWeightOperator::setup(std::shared_ptr<CBCT_acquisition> acq,
                                    std::shared_ptr<CBCT_binning> binning, floatd3 is_dims_in_mm,
@@ -373,7 +396,7 @@ void CBSubsetWeightOperator::setup(std::shared_ptr<CBCT_acquisition> acq,
   // the weights in the projections' permuted order (permute_projections)
   auto dims = projection_weights->dims();
   dims.back() = permutations_.size();
-  hoCuNDArray<float> permuted(dims);
+  hoCuNDArray<float> permuted(dims, no_init);
   const size_t plane = projection_weights->get_size(0) * projection_weights->get_size(1);
   for (size_t i = 0; i < permutations_.size(); ++i)
     std::copy(projection_weights->get_data_ptr() + plane * permutations_[i],

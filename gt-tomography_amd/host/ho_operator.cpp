@@ -1,5 +1,4 @@
-// hoCuConebeamProjectionOperator (xray/hoCuConebeamProjectionOperator.{h,cpp})
-// and the FDK_reconstruct_3d driver logic (xray/FDK_reconstruct_3d.cpp:128-161).
+// hoCuConebeamProjectionOperator (xray/hoCuConebeamProjectionOperator.{h,cpp}).
 // Host arrays in and out; each call stages the bin's data in HBM, runs the
 // gfx950 kernels and copies the result back.  The reference streams 20-view
 // batches through two small device buffers; with 288 GB of HBM a whole bin is
@@ -237,33 +236,6 @@ void hoCuConebeamProjectionOperator::offset_correct(hoCuNDArray<float> *projecti
   const cbct_geom g = geom_for(1, 1, 1, projections->get_size(0), projections->get_size(1));
   check(cbct_offset_weights(ctx_, &g, 0, (int)nv, d.get_data_ptr(), 1, current_stream()), ctx_);
   d.to_host(projections);
-}
-
-FDKResult FDK_reconstruct_3d(std::shared_ptr<CBCT_acquisition> acquisition,
-                             std::shared_ptr<CBCT_binning> binning,
-                             const std::vector<size_t> &is_dims, floatd3 is_dims_in_mm,
-                             std::shared_ptr<std::vector<float>> frequency_filter,
-                             float samples_per_pixel) {
-  if (!binning) {
-    binning = std::make_shared<CBCT_binning>();
-    binning->set_as_default_3d_bin((unsigned int)acquisition->get_projections()->get_size(2));
-  }
-  FDKResult r;
-  r.image = std::make_shared<hoCuNDArray<float>>(is_dims);
-  hoCuNDArray<float> projections(*acquisition->get_projections());
-  hoCuConebeamProjectionOperator E;
-  E.set_num_samples_per_pixel(samples_per_pixel);
-  E.setup(acquisition, binning, is_dims_in_mm);
-  E.set_use_filtered_backprojection(true);
-  if (frequency_filter) E.set_frequency_filter(frequency_filter);
-  E.mult_MH(&projections, r.image.get());
-  hoCuNDArray<float> tmp_proj(projections.dims());
-  E.mult_M(r.image.get(), &tmp_proj, false);
-  r.scale = dot(&tmp_proj, &projections) / dot(&tmp_proj, &tmp_proj);
-  float *p = r.image->get_data_ptr();
-  const float s = float(r.scale);
-  for (size_t i = 0, n = r.image->get_number_of_elements(); i < n; ++i) p[i] *= s;
-  return r;
 }
 
 }  // namespace gtt

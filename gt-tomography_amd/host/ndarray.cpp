@@ -5,6 +5,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <map>
+#include <vector>
 #include <string>
 
 namespace gtt {
@@ -39,6 +40,7 @@ void set_default_storage(int dtype) { t_storage = dtype == CBCT_BF16 ? CBCT_BF16
 int default_storage() { return t_storage; }
 void set_current_stream(void *stream) { t_stream = static_cast<hipStream_t>(stream); }
 void release_thread_resources() {
+  trim_device_cache();
   for (auto &kv : t_ctx) cbct_ctx_destroy(kv.second);
   t_ctx.clear();
 }
@@ -47,8 +49,69 @@ void check(int rc, cbct_ctx ctx) {
   if (rc != CBCT_OK) throw std::runtime_error(cbct_last_error(ctx));
 }
 
-void device_malloc(void **p, size_t bytes) { hip_check(hipMalloc(p, bytes), "hipMalloc"); }
-void device_free(void *p) { (void)hipFree(p); }
+// Freed device blocks are kept for reuse by the same thread on the same
+// device and stream: the operators and solvers create and drop arrays of the
+// same few shapes every subset, and hipFree synchronises the device and costs
+// milliseconds on large blocks.  Work queued on the freeing stream is ordered
+// before any later use of the block on that stream, so reuse needs no wait.
+namespace {
+struct CachedBlock {
+  void *p;
+  size_t bytes;
+  int device;
+  hipStream_t stream;
+};
+thread_local std::vector<CachedBlock> t_free_blocks;
+thread_local std::map<void *, size_t> t_block_sizes;   // live blocks from device_malloc
+thread_local size_t t_cached_bytes = 0;
+constexpr size_t kCacheLimit = size_t(32) << 30;
+}  // namespace
+
+void trim_device_cache() {
+  for (auto &b : t_free_blocks) (void)hipFree(b.p);
+  t_free_blocks.clear();
+  t_cached_bytes = 0;
+}
+
+void device_malloc(void **p, size_t bytes) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  for (size_t i = 0; i < t_free_blocks.size(); ++i) {
+    const CachedBlock &b = t_free_blocks[i];
+    if (b.bytes == bytes && b.device == dev && b.stream == t_stream) {
+      *p = b.p;
+      t_cached_bytes -= bytes;
+      t_free_blocks.erase(t_free_blocks.begin() + i);
+      t_block_sizes[*p] = bytes;
+      return;
+    }
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && !t_free_blocks.empty()) {
+    (void)hipGetLastError();
+    trim_device_cache();
+    e = hipMalloc(p, bytes);
+  }
+  hip_check(e, "hipMalloc");
+  t_block_sizes[*p] = bytes;
+}
+
+void device_free(void *p) {
+  auto it = t_block_sizes.find(p);
+  if (it == t_block_sizes.end()) {   // not this thread's block
+    (void)hipFree(p);
+    return;
+  }
+  const size_t bytes = it->second;
+  t_block_sizes.erase(it);
+  int dev = 0;
+  if (t_cached_bytes + bytes > kCacheLimit || hipGetDevice(&dev) != hipSuccess) {
+    (void)hipFree(p);
+    return;
+  }
+  t_free_blocks.push_back({p, bytes, dev, t_stream});
+  t_cached_bytes += bytes;
+}
 void copy_d2d(void *dst, const void *src, size_t bytes) {
   hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, t_stream), "hipMemcpyAsync");
 }

@@ -20,10 +20,12 @@ struct Scan {
 
 // bin_index holds the concatenated view indices of nbins bins with sizes
 // bin_sizes; nbins <= 0 selects one bin of all views.  proj (nullable) is
-// copied into the acquisition's host projection array.
+// copied into the acquisition's host projection array, or, with borrow, the
+// array wraps the caller's buffer (then nothing may write to it: the subset
+// operators are set up without the host permutation).
 inline Scan make_scan(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
                       const unsigned *bin_index, const int *bin_sizes, int nbins,
-                      const float *proj) {
+                      const float *proj, bool borrow = false) {
   if (!g || !angles || nviews <= 0) throw std::runtime_error("illegal geometry or angle array provided");
   Scan s;
   auto geom = std::make_shared<CBCT_geometry>();
@@ -37,8 +39,13 @@ inline Scan make_scan(const cbct_geom *g, const float *angles, const float *offs
   geom->set_offsets(o);
   std::shared_ptr<hoCuNDArray<float>> p;
   if (proj) {
-    p = std::make_shared<hoCuNDArray<float>>(std::vector<size_t>{(size_t)g->nu, (size_t)g->nv, (size_t)nviews});
-    std::memcpy(p->get_data_ptr(), proj, p->get_number_of_bytes());
+    const std::vector<size_t> dims{(size_t)g->nu, (size_t)g->nv, (size_t)nviews};
+    if (borrow) {
+      p = std::make_shared<hoCuNDArray<float>>(dims, const_cast<float *>(proj));
+    } else {
+      p = std::make_shared<hoCuNDArray<float>>(dims, no_init);
+      std::memcpy(p->get_data_ptr(), proj, p->get_number_of_bytes());
+    }
   }
   s.acq = std::make_shared<CBCT_acquisition>(p, geom);
   s.binning = std::make_shared<CBCT_binning>();

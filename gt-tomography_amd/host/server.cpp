@@ -74,7 +74,8 @@ struct StorageScope {
 void run_request(Job &j) {
   const gtt_request &r = j.req;
   StorageScope storage(r.storage == CBCT_BF16 ? CBCT_BF16 : CBCT_F32);
-  Scan s = make_scan(&r.geom, r.angles_deg, r.offsets, r.nviews, nullptr, nullptr, 0, r.projections);
+  // the caller's projections are only read (uploaded once), never copied
+  Scan s = make_scan(&r.geom, r.angles_deg, r.offsets, r.nviews, nullptr, nullptr, 0, r.projections, true);
   if (r.kind == GTT_REQ_FDK) {
     std::shared_ptr<std::vector<float>> F;
     if (r.filter) F = std::make_shared<std::vector<float>>(r.filter, r.filter + r.geom.nu + 1);
@@ -85,9 +86,9 @@ void run_request(Job &j) {
   } else {
     if (r.subsets <= 0) throw std::runtime_error("number of subsets must be positive");
     auto E = std::make_shared<CBSubsetOperator>(r.subsets);
-    E->setup(s.acq, s.binning, s.dims_mm);
+    E->setup(s.acq, s.binning, s.dims_mm, false);
     E->set_domain_dimensions(&s.vol_dims);
-    cuNDArray<float> b(*s.acq->get_projections());
+    cuNDArray<float> b = upload_in_subset_order(*s.acq->get_projections(), E->get_permutation());
     osSARTSolver<cuNDArray<float>> solver;
     solver.set_encoding_operator(E);
     solver.set_max_iterations(r.iterations);
@@ -152,6 +153,8 @@ struct gtt_server_s {
           status = -1;
           msg = e.what();
         }
+        // the request's cached device blocks go back before its footprint does
+        trim_device_cache();
       }
       {
         std::lock_guard<std::mutex> lk(mu);

@@ -21,7 +21,10 @@ struct FDKResult {
 
 // acquisition angles are shifted in place by the operator setup, as in the
 // reference.  frequency_filter: nu+1 one-sided taps (nullptr = the default
-// filter of compute_default_frequency_filter).
+// filter of compute_default_frequency_filter).  One FBP volume per bin
+// ((nx, ny, nz, nbins) when nbins > 1); the projections are only read.
+// Runs device-resident (the reference's hoCu operator streams through host
+// memory and forms the inner products on the host); same results.
 FDKResult FDK_reconstruct_3d(std::shared_ptr<CBCT_acquisition> acquisition,
                              std::shared_ptr<CBCT_binning> binning,
                              const std::vector<size_t> &is_dims, floatd3 is_dims_in_mm,

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstddef>
+#include <algorithm>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -39,6 +40,9 @@ void check(int rc, cbct_ctx ctx);
 
 void device_malloc(void **p, size_t bytes);
 void device_free(void *p);
+// Frees the device blocks the calling thread keeps for reuse (device_free
+// caches blocks per thread, device and stream).
+void trim_device_cache();
 void copy_d2d(void *dst, const void *src, size_t bytes);
 void copy_h2d(void *dst, const void *src, size_t bytes);
 void copy_d2h(void *dst, const void *src, size_t bytes);
@@ -125,22 +129,46 @@ class cuNDArray {
   bool owns_ = false;
 };
 
-// Host array (the reference's page-locked hoCuNDArray).
+// Host array (the reference's page-locked hoCuNDArray).  Owning arrays are
+// zero-filled unless made with `no_init` (for arrays about to be overwritten
+// whole); the (dims, data) form wraps caller memory without owning it.
+struct no_init_t {};
+constexpr no_init_t no_init{};
+
 template <class T>
 class hoCuNDArray {
  public:
   typedef T element_type;
   hoCuNDArray() = default;
-  explicit hoCuNDArray(const std::vector<size_t> &dims) : dims_(dims), own_(prod(dims)), data_(own_.data()) {}
+  explicit hoCuNDArray(const std::vector<size_t> &dims)
+      : dims_(dims), own_(new T[prod(dims)]()), data_(own_.get()) {}
+  hoCuNDArray(const std::vector<size_t> &dims, no_init_t)
+      : dims_(dims), own_(new T[prod(dims)]), data_(own_.get()) {}
   explicit hoCuNDArray(const std::vector<size_t> *dims) : hoCuNDArray(*dims) {}
   explicit hoCuNDArray(std::shared_ptr<std::vector<size_t>> dims) : hoCuNDArray(*dims) {}
   hoCuNDArray(const std::vector<size_t> &dims, T *data) : dims_(dims), data_(data) {}
-  hoCuNDArray(const hoCuNDArray &o) : dims_(o.dims_), own_(o.data_, o.data_ + o.get_number_of_elements()), data_(own_.data()) {}
+  hoCuNDArray(const hoCuNDArray &o)
+      : dims_(o.dims_), own_(new T[o.get_number_of_elements()]), data_(own_.get()) {
+    std::copy(o.data_, o.data_ + o.get_number_of_elements(), data_);
+  }
+  hoCuNDArray(hoCuNDArray &&o) noexcept
+      : dims_(std::move(o.dims_)), own_(std::move(o.own_)), data_(o.data_) {
+    o.data_ = nullptr;
+  }
   hoCuNDArray &operator=(const hoCuNDArray &o) {
     if (this == &o) return *this;
+    std::unique_ptr<T[]> n(new T[o.get_number_of_elements()]);
+    std::copy(o.data_, o.data_ + o.get_number_of_elements(), n.get());
     dims_ = o.dims_;
-    own_.assign(o.data_, o.data_ + o.get_number_of_elements());
-    data_ = own_.data();
+    own_ = std::move(n);
+    data_ = own_.get();
+    return *this;
+  }
+  hoCuNDArray &operator=(hoCuNDArray &&o) noexcept {
+    dims_ = std::move(o.dims_);
+    own_ = std::move(o.own_);
+    data_ = o.data_;
+    o.data_ = nullptr;
     return *this;
   }
   std::shared_ptr<std::vector<size_t>> get_dimensions() const {
@@ -155,7 +183,7 @@ class hoCuNDArray {
 
  private:
   std::vector<size_t> dims_;
-  std::vector<T> own_;
+  std::unique_ptr<T[]> own_;
   T *data_ = nullptr;
 };
 

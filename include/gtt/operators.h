@@ -239,9 +239,12 @@ class CBSubsetOperator : public subsetOperator<cuNDArray<float>> {
     for (auto &op : operators_) op->set_domain_dimensions(d);
   }
   // :52-158 -- round-robin split, permutation of projections/angles/offsets,
-  // per-subset projector setup (transform_angles = false)
+  // per-subset projector setup (transform_angles = false).  With
+  // permute_projections = false (no reference counterpart) the acquisition's
+  // host projections are left in acquisition order, for callers that upload
+  // them once and permute on the device (upload_in_subset_order).
   void setup(std::shared_ptr<CBCT_acquisition> acq, std::shared_ptr<CBCT_binning> binning,
-             floatd3 is_dims_in_mm);
+             floatd3 is_dims_in_mm, bool permute_projections = true);
   void setup(std::shared_ptr<CBCT_acquisition> acq, floatd3 is_dims_in_mm);
   void offset_correct(cuNDArray<float> *projections);
   void set_use_offset_correction(bool use) {
@@ -261,6 +264,13 @@ class CBSubsetOperator : public subsetOperator<cuNDArray<float>> {
   std::vector<std::shared_ptr<std::vector<size_t>>> projection_dims_;
   std::vector<unsigned int> permutations_;
 };
+
+// Host projections in acquisition order, uploaded once and put in the
+// subset operator's order (its permutation) on the device.  No reference
+// counterpart: the reference permutes on the host (CBSubsetOperator.cpp:106,
+// :192-212) and uploads the permuted array.
+cuNDArray<float> upload_in_subset_order(const hoCuNDArray<float> &projections,
+                                        const std::vector<unsigned int> &permutation);
 
 // ---------------------------------------------------------------------------
 // Per-projection statistical weights around the subset projector
