@@ -35,6 +35,10 @@
 // volume, 8 gathers per sample.  Also the basis of the exact transpose.
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "cbct_internal.h"
 
@@ -878,31 +882,60 @@ FPChoice default_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, in
   return {per_step, rows_per_lane(P.nv, ctx->fp_max_rpl)};
 }
 
+// Autotuned choices, process-wide (cbct_fp_autotune): the geometry, the
+// samples per ray and the working-copy element size pick (variant, tile).
+struct TuneKey {
+  int nx, ny, nz, nu, nv, esz;
+  float lx, ly, lz, fu, fv, sad, sdd, spp;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(nx, ny, nz, nu, nv, esz, lx, ly, lz, fu, fv, sad, sdd, spp) <
+           std::tie(o.nx, o.ny, o.nz, o.nu, o.nv, o.esz, o.lx, o.ly, o.lz, o.fu, o.fv, o.sad, o.sdd,
+                    o.spp);
+  }
+};
+struct Tuned {
+  FPChoice choice;
+  double ms;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, Tuned> g_tuned;
+
+TuneKey tune_key(const cbct_geom *g, float spp, size_t esz) {
+  return {g->nx, g->ny, g->nz, g->nu, g->nv, (int)esz, g->lx, g->ly, g->lz, g->fov_u, g->fov_v,
+          g->sad, g->sdd, spp};
+}
+
 // The variant cbct_fp runs and its tile: variant 0 = per ray (engine 0, or a
 // volume or z-line beyond the column engine's 32-bit byte offsets / 64 KB of
 // LDS); engine 1: default_variant; engine 2: the other per-step variant;
 // engines 3 and 4: the cached variants (configured tile).
 FPChoice select_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
-                        size_t esz) {
+                        size_t esz, float spp) {
   const size_t vt_bytes = (size_t)P.pzx * padded_dim(g->ny) * esz;
   const size_t zline_bytes = ((P.pz + 127) / 128) * 128 * sizeof(float);
   const int rpl = rows_per_lane(P.nv, ctx->fp_max_rpl);
   if (ctx->fp_engine == 0 || vt_bytes + (size_t)P.pzx * esz * 2 >= (1ull << 31) || zline_bytes > 65536)
     return {0, rpl};
+  if (ctx->fp_force_variant > 0) return {ctx->fp_force_variant, ctx->fp_force_rpl};
   const int per_step = esz == 2 ? 2 : 1;
   if (ctx->fp_engine == 2) return {3 - per_step, rpl};
   if (ctx->fp_engine >= 3) return {ctx->fp_engine, rpl};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(tune_key(g, spp, esz));
+    if (it != g_tuned.end()) return it->second.choice;
+  }
   return default_variant(ctx, g, P, view0, nviews, per_step);
 }
 
 // TI: the caller's volume type; T: the padded working copy's type.
 template <typename TI, typename T>
 int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
-           const TI *vol, float *proj, int accumulate, hipStream_t s) {
+           const TI *vol, float *proj, int accumulate, hipStream_t s, float spp) {
   const size_t esz = sizeof(T);
   const ViewD *views = ctx->d_views_d + view0;
   const size_t vt_bytes = (size_t)P.pzx * padded_dim(g->ny) * esz;
-  const FPChoice choice = select_variant(ctx, g, P, view0, nviews, esz);
+  const FPChoice choice = select_variant(ctx, g, P, view0, nviews, esz, spp);
   if (choice.variant == 0) {
     const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
     int rc = cbct_reserve(ctx, ctx->vol_pad, npad * esz);
@@ -950,6 +983,108 @@ int cbct_fp_engine(cbct_ctx ctx, int engine) {
   return CBCT_OK;
 }
 
+int cbct_fp_autotune(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, int dtype, float spp,
+                     int retune, void *stream, double *best_ms) {
+  if (!ctx) return CBCT_ERR_ARG;
+  const char *who = "cbct_fp_autotune";
+  int rc = cbct_check_geom(ctx, g, who);
+  if (rc) return rc;
+  rc = cbct_check_views(ctx, view0, nviews, who);
+  if (rc) return rc;
+  if (nviews <= 0) return cbct_fail(ctx, CBCT_ERR_ARG, "%s: no views", who);
+  if (dtype != CBCT_F32 && dtype != CBCT_BF16)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "%s: unsupported volume dtype %d", who, dtype);
+  FPParams P;
+  rc = make_params(ctx, g, spp, P);
+  if (rc) return rc;
+  rc = cbct_set_device(ctx);
+  if (rc) return rc;
+  const size_t esz = (dtype == CBCT_BF16 || ctx->work_dtype == CBCT_BF16) ? 2 : 4;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(tune_key(g, spp, esz));
+    if (it != g_tuned.end()) {
+      if (!retune) {
+        if (best_ms) *best_ms = it->second.ms;
+        return CBCT_OK;
+      }
+      g_tuned.erase(it);   // time the built-in choice itself
+    }
+  }
+  const int saved_engine = ctx->fp_engine;
+  ctx->fp_engine = 1;
+  const FPChoice heuristic = select_variant(ctx, g, P, view0, nviews, esz, spp);
+  ctx->fp_engine = saved_engine;
+  if (heuristic.variant == 0) return CBCT_OK;   // the column engines do not apply
+  // candidates: the heuristic's; the cached variants with 2- and 3-chunk
+  // windows at several tiles; the per-step variant of the storage type
+  std::vector<FPChoice> cand{heuristic};
+  auto add = [&](int v, int r) {
+    for (auto &c : cand)
+      if (c.variant == v && c.rpl == r) return;
+    cand.push_back({v, r});
+  };
+  for (int m : {4, 6, 8, 12}) add(3, rows_per_lane(P.nv, m));
+  for (int m : {6, 8}) add(4, rows_per_lane(P.nv, m));
+  add(esz == 2 ? 2 : 1, rows_per_lane(P.nv, 8));
+  // scratch volume and projections (contents do not change the timing)
+  const size_t nvox = (size_t)g->nx * g->ny * g->nz;
+  const size_t nproj = (size_t)g->nu * g->nv * (size_t)nviews;
+  void *vol = nullptr, *proj = nullptr;
+  CBCT_HIP_CHECK(ctx, hipMalloc(&vol, nvox * (dtype == CBCT_BF16 ? 2 : 4)));
+  if (hipMalloc(&proj, nproj * sizeof(float)) != hipSuccess) {
+    (void)hipFree(vol);
+    (void)hipGetLastError();
+    return cbct_fail(ctx, CBCT_ERR_NOMEM, "%s: scratch allocation failed", who);
+  }
+  hipStream_t s = as_stream(stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const size_t vol_words = nvox * (dtype == CBCT_BF16 ? 2 : 4) / 4;   // whole 4-byte words
+  hipLaunchKernelGGL(k_zero, stride_grid(vol_words), dim3(256), 0, s, vol_words, static_cast<float *>(vol));
+  FPChoice best = heuristic;
+  double best_t = 1e300;
+  for (size_t c = 0; c < cand.size() && rc == CBCT_OK; ++c) {
+    ctx->fp_force_variant = cand[c].variant;
+    ctx->fp_force_rpl = cand[c].rpl;
+    double t = 1e300;
+    for (int rep = 0; rep < 3 && rc == CBCT_OK; ++rep) {   // first run warms up
+      (void)hipEventRecord(e0, s);
+      rc = cbct_fp(ctx, g, view0, nviews, vol, dtype, static_cast<float *>(proj), spp, 0, stream);
+      (void)hipEventRecord(e1, s);
+      if (rc == CBCT_OK && hipEventSynchronize(e1) == hipSuccess && rep > 0) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) t = std::min(t, (double)ms);
+      }
+    }
+    if (t < best_t) {
+      best_t = t;
+      best = cand[c];
+    }
+  }
+  ctx->fp_force_variant = 0;
+  ctx->fp_force_rpl = 0;
+  (void)hipStreamSynchronize(s);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(vol);
+  (void)hipFree(proj);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[tune_key(g, spp, esz)] = {best, best_t};
+  }
+  if (best_ms) *best_ms = best_t;
+  return CBCT_OK;
+}
+
+int cbct_fp_autotune_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned.clear();
+  return CBCT_OK;
+}
+
 int cbct_fp_tile(cbct_ctx ctx, int max_rows_per_lane) {
   if (!ctx) return CBCT_ERR_ARG;
   if (max_rows_per_lane < 1 || max_rows_per_lane > 12)
@@ -966,7 +1101,7 @@ const char *cbct_fp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nvie
   FPParams P;
   if (make_params(ctx, g, spp, P)) return "";
   const size_t esz = (dtype == CBCT_BF16 || ctx->work_dtype == CBCT_BF16) ? 2 : 4;
-  const int v = select_variant(ctx, g, P, view0, nviews, esz).variant;
+  const int v = select_variant(ctx, g, P, view0, nviews, esz, spp).variant;
   return v == 0 ? "k_fp_ray" : v >= 3 ? "k_fp_colc" : "k_fp_col";
 }
 
@@ -996,12 +1131,12 @@ int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
   if (dtype == CBCT_BF16)
     return run_fp<unsigned short, unsigned short>(ctx, g, P, view0, nviews,
                                                   static_cast<const unsigned short *>(vol), proj,
-                                                  accumulate, s);
+                                                  accumulate, s, spp);
   if (ctx->work_dtype == CBCT_BF16)
     return run_fp<float, unsigned short>(ctx, g, P, view0, nviews, static_cast<const float *>(vol),
-                                         proj, accumulate, s);
+                                         proj, accumulate, s, spp);
   return run_fp<float, float>(ctx, g, P, view0, nviews, static_cast<const float *>(vol), proj,
-                              accumulate, s);
+                              accumulate, s, spp);
 }
 
 int cbct_fp_transpose(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *proj,

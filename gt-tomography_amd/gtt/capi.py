@@ -103,6 +103,8 @@ def load():
         "cbct_fp_transpose": (I, [P, G, I, I, P, P, F, I, P]),
         "cbct_fp_engine": (I, [P, I]),
         "cbct_fp_kernel": (ctypes.c_char_p, [P, G, I, I, I, F]),
+        "cbct_fp_autotune": (I, [P, G, I, I, I, F, I, P, ctypes.POINTER(ctypes.c_double)]),
+        "cbct_fp_autotune_clear": (I, []),
         "cbct_fp_tile": (I, [P, I]),
         "cbct_bp_engine": (I, [P, I]),
         "cbct_storage": (I, [P, I]),
@@ -254,6 +256,17 @@ class Context:
         code = CBCT_BF16 if dtype in ("bf16", "bfloat16") else CBCT_F32
         return self.lib.cbct_fp_kernel(self.h, ctypes.byref(g), view0, n, code,
                                        samples_per_pixel).decode()
+
+    def fp_autotune(self, g: Geom, view0=0, nviews=None, dtype="f32", samples_per_pixel=1.5,
+                    retune=False, stream=None):
+        """Time the column-engine candidates for this geometry (unless it was
+        tuned before); engine 1 then uses the fastest.  Returns its time in ms."""
+        n = self.nviews() - view0 if nviews is None else nviews
+        code = CBCT_BF16 if dtype in ("bf16", "bfloat16") else CBCT_F32
+        out = ctypes.c_double()
+        self._check(self.lib.cbct_fp_autotune(self.h, ctypes.byref(g), view0, n, code, samples_per_pixel,
+                                              int(retune), _stream(stream), ctypes.byref(out)))
+        return out.value
 
     def fp_engine(self, engine):
         """0 = one thread per ray, >= 1 = detector-column sharing variants (1 default)."""
@@ -428,6 +441,10 @@ class Context:
         self._check(self.lib.cbct_mssim(self.h, nx, ny, nz, _ptr(image), _ptr(reference), sx, sy, sz,
                                         int(scales), ctypes.byref(out), _stream(stream)))
         return out.value
+
+
+def fp_autotune_clear():
+    load().cbct_fp_autotune_clear()
 
 
 def cosine_weights(g: Geom):

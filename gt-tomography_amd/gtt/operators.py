@@ -34,6 +34,8 @@ def load():
     L.gtt_last_error.argtypes = []
     L.gtt_set_default_storage.argtypes = [I]
     L.gtt_set_default_storage.restype = I
+    L.gtt_set_autotune.argtypes = [I]
+    L.gtt_set_autotune.restype = I
     L.gtt_fdk_reconstruct_3d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, D]
     L.gtt_fdk_reconstruct_4d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, P, D]
     L.gtt_os_sart.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, P, D]
@@ -60,6 +62,12 @@ def set_default_storage(dtype):
     """Working copies of the projectors ('f32' or 'bf16') for reconstructions
     this thread starts from now on."""
     _check(load().gtt_set_default_storage(capi.CBCT_BF16 if dtype == "bf16" else capi.CBCT_F32))
+
+
+def set_autotune(on):
+    """OS reconstructions this thread starts from now on tune the forward
+    projector for their geometry first."""
+    _check(load().gtt_set_autotune(int(bool(on))))
 
 
 def _check(rc):

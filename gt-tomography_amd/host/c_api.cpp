@@ -88,6 +88,11 @@ int gtt_set_default_storage(int dtype) {
   return 0;
 }
 
+int gtt_set_autotune(int on) {
+  set_default_autotune(on != 0);
+  return 0;
+}
+
 int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
                            const unsigned *bin_index, const int *bin_sizes, int nbins,
                            const float *projections, const float *filter, float spp,
@@ -141,6 +146,7 @@ int gtt_os_sart_weighted(const cbct_geom *g, const float *angles, const float *o
                        weights == nullptr);
     auto E = make_subset_operator(nsubsets, s, weights);
     E->set_domain_dimensions(&s.vol_dims);
+    if (default_autotune()) E->autotune();
     cuNDArray<float> b = subset_data(*E, s, weights);
     osSARTSolver<cuNDArray<float>> solver;
     solver.set_encoding_operator(E);
@@ -176,6 +182,7 @@ int gtt_os_mom_weighted(const cbct_geom *g, const float *angles, const float *of
                        weights == nullptr);
     auto E = make_subset_operator(nsubsets, s, weights);
     E->set_domain_dimensions(&s.vol_dims);
+    if (default_autotune()) E->autotune();
     cuNDArray<float> b = subset_data(*E, s, weights);
     osMOMSolverD<cuNDArray<float>> solver;
     solver.set_dump(false);

@@ -173,6 +173,21 @@ void cuConebeamProjectionOperator::mult_M(cuNDArray<float> *input, cuNDArray<flo
   }
 }
 
+void cuConebeamProjectionOperator::autotune(size_t nu, size_t nv) {
+  if (!preprocessed_) throw std::runtime_error("Error: cuConebeamProjectionOperator::autotune: setup not performed");
+  std::vector<size_t> dims3d = domain_;
+  if (dims3d.size() == 4) dims3d.pop_back();
+  if (dims3d.size() != 3)
+    throw std::runtime_error("Error: cuConebeamProjectionOperator::autotune: domain dimensions not set");
+  size_t best = 0;
+  for (unsigned int b = 1; b < binning_->get_number_of_bins(); ++b)
+    if (angles_[b].size() > angles_[best].size()) best = b;
+  if (angles_.empty() || angles_[best].empty()) return;
+  const cbct_geom g = geom_for(dims3d[0], dims3d[1], dims3d[2], nu, nv);
+  check(cbct_fp_autotune(ctx_, &g, bin_view0_[best], (int)angles_[best].size(), CBCT_F32,
+                         samples_per_pixel_, 0, current_stream(), nullptr), ctx_);
+}
+
 void cuConebeamProjectionOperator::mult_MH(cuNDArray<float> *input, cuNDArray<float> *output,
                                            bool accumulate) {
   if (!input || !output)

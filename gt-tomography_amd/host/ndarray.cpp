@@ -17,6 +17,7 @@ namespace {
 thread_local std::map<int, cbct_ctx> t_ctx;
 thread_local hipStream_t t_stream = nullptr;
 thread_local int t_storage = CBCT_F32;
+thread_local bool t_autotune = false;
 
 void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -38,6 +39,8 @@ cbct_ctx device_context() {
 void *current_stream() { return t_stream; }
 void set_default_storage(int dtype) { t_storage = dtype == CBCT_BF16 ? CBCT_BF16 : CBCT_F32; }
 int default_storage() { return t_storage; }
+void set_default_autotune(bool on) { t_autotune = on; }
+bool default_autotune() { return t_autotune; }
 void set_current_stream(void *stream) { t_stream = static_cast<hipStream_t>(stream); }
 void release_thread_resources() {
   trim_device_cache();

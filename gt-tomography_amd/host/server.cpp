@@ -48,6 +48,7 @@ struct Job {
   bool done = false;
   int status = 0;
   double scale = 0.0;
+  bool autotune = false;   // tune the forward projector for the scan's geometry first
   std::string message;
 };
 
@@ -88,6 +89,7 @@ void run_request(Job &j) {
     auto E = std::make_shared<CBSubsetOperator>(r.subsets);
     E->setup(s.acq, s.binning, s.dims_mm, false);
     E->set_domain_dimensions(&s.vol_dims);
+    if (j.autotune) E->autotune();
     cuNDArray<float> b = upload_in_subset_order(*s.acq->get_projections(), E->get_permutation());
     osSARTSolver<cuNDArray<float>> solver;
     solver.set_encoding_operator(E);
@@ -104,6 +106,7 @@ void run_request(Job &j) {
 
 struct gtt_server_s {
   int device = 0;
+  bool autotune = false;
   size_t budget = 0;
   size_t in_use = 0;
   int running = 0;
@@ -211,6 +214,13 @@ const char *gtt_server_last_error(gtt_server s) { return s ? s->err.c_str() : "n
 
 size_t gtt_request_footprint(const gtt_request *req) { return req ? footprint_bytes(*req) : 0; }
 
+int gtt_server_set_autotune(gtt_server s, int on) {
+  if (!s) return -1;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->autotune = on != 0;
+  return 0;
+}
+
 int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id) {
   if (!s) return -1;
   std::lock_guard<std::mutex> lk(s->mu);
@@ -232,6 +242,7 @@ int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id) {
   }
   auto j = std::make_shared<Job>();
   j->req = *req;
+  j->autotune = s->autotune;
   j->footprint = footprint_bytes(*req);
   if (j->footprint > s->budget) {
     s->err = "gtt_server_submit: request needs " + std::to_string(j->footprint) +

@@ -107,6 +107,20 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
  * counterpart (the reference has one texture-based kernel). */
 int cbct_fp_engine(cbct_ctx ctx, int engine);
 
+/* Forward-projector autotuner: times candidate column-engine variants and
+ * tiles (each once warm, twice timed) for views [view0, view0+nviews) of
+ * this geometry on scratch arrays, and records the fastest for the geometry,
+ * samples per ray and working-copy type in a process-wide table that engine
+ * 1 consults before its built-in choice.  A geometry already in the table
+ * returns at once unless retune != 0.  All candidates give the same bits.
+ * Time it on an otherwise idle device.  Synchronises the stream; allocates
+ * and frees its scratch.  best_ms (nullable): the winner's time.  No
+ * reference counterpart. */
+int cbct_fp_autotune(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, int vol_dtype,
+                     float samples_per_pixel, int retune, void *stream, double *best_ms);
+/* Forgets every tuned choice. */
+int cbct_fp_autotune_clear(void);
+
 /* Name of the kernel cbct_fp would launch for these arguments under the
  * context's current settings ("k_fp_ray", "k_fp_col" or "k_fp_colc"; "" on
  * bad arguments), for matching profiler output.  No reference counterpart. */

@@ -33,6 +33,10 @@ void release_thread_resources();
 // projection operators the calling thread creates from now on (default FP32).
 void set_default_storage(int dtype);
 int default_storage();
+// Whether reconstructions the calling thread starts tune the forward
+// projector for their geometry first (cbct_fp_autotune; default off).
+void set_default_autotune(bool on);
+bool default_autotune();
 // HIP device current on the calling thread.
 int current_device();
 // Throw std::runtime_error with the library's message when rc != 0.

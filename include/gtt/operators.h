@@ -139,6 +139,10 @@ class cuConebeamProjectionOperator : public linearOperator<cuNDArray<float>> {
   // Working-copy storage of the projectors (CBCT_F32 / CBCT_BF16); new
   // operators take the creating thread's default_storage().
   void set_storage(int dtype) { check(cbct_storage(ctx_, dtype), ctx_); }
+  // Tunes the forward projector for this operator's geometry (domain
+  // dimensions set, detector nu x nv) on its largest bin (cbct_fp_autotune).
+  // No reference counterpart.
+  void autotune(size_t nu, size_t nv);
 
  protected:
   cbct_geom geom_for(size_t nx, size_t ny, size_t nz, size_t nu, size_t nv) const;
@@ -255,6 +259,13 @@ class CBSubsetOperator : public subsetOperator<cuNDArray<float>> {
   }
   void set_storage(int dtype) {
     for (auto &op : operators_) op->set_storage(dtype);
+  }
+  // Tunes the forward projector for the subsets' geometry (after
+  // set_domain_dimensions); the choice is shared by every operator with the
+  // same geometry.  No reference counterpart.
+  void autotune() {
+    if (!operators_.empty() && !projection_dims_.empty())
+      operators_[0]->autotune(projection_dims_[0]->at(0), projection_dims_[0]->at(1));
   }
   const std::vector<unsigned int> &get_permutation() const { return permutations_; }
   std::shared_ptr<cuConebeamProjectionOperator> get_subset_operator(int s) { return operators_[s]; }

@@ -37,6 +37,12 @@ const char *gtt_last_error(void);
  * cbct_storage).  No reference counterpart. */
 int gtt_set_default_storage(int dtype);
 
+/* Whether the OS reconstructions the calling thread starts from now on tune
+ * the forward projector for their geometry first (cbct_fp_autotune; tuned
+ * choices are kept per geometry for the process).  Default off.  No
+ * reference counterpart. */
+int gtt_set_autotune(int on);
+
 int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles_deg, const float *offsets,
                            int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
                            const float *projections, const float *filter /* nu+1 or NULL */,
@@ -143,6 +149,10 @@ typedef struct {
 typedef struct gtt_server_s *gtt_server;
 int gtt_server_create(int device, int lanes, size_t hbm_budget_bytes, gtt_server *out);
 int gtt_server_destroy(gtt_server s);  /* drains the queue first */
+/* OS-SART requests submitted from now on tune the forward projector for
+ * their geometry first (cbct_fp_autotune; choices are kept per geometry for
+ * the process, so repeated geometries pay once).  Default off. */
+int gtt_server_set_autotune(gtt_server s, int on);
 int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id);
 int gtt_server_wait(gtt_server s, long long id, gtt_result *res);
 const char *gtt_server_last_error(gtt_server s);

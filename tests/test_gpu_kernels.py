@@ -444,3 +444,26 @@ def test_errors_are_reported(ctx):
         c2.set_views(angles_full(4))
         c2.fdk_prefilter(g, out, out)
     c2.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_fp_autotune_keeps_bits(ctx, dtype):
+    """The autotuner (cbct_fp_autotune) times column-engine variants and tiles
+    and records the fastest for the geometry; projections afterwards are
+    bitwise those of the built-in choice."""
+    g, angles, offsets = make_case("ragged")
+    print_case("autotune", dtype, g)
+    capi.fp_autotune_clear()
+    vol = phantom_volume(g) + 0.01
+    tdt = torch.float32 if dtype == "f32" else torch.bfloat16
+    ctx.set_views(angles, offsets)
+    before = torch.zeros(g.proj_shape(len(angles)), device="cuda")
+    ctx.fp(g, dev(vol, tdt), before)
+    ms = ctx.fp_autotune(g, 0, len(angles), dtype)
+    assert ms > 0.0
+    assert ctx.fp_kernel(g, 0, len(angles), dtype) in ("k_fp_col", "k_fp_colc")
+    after = torch.zeros_like(before)
+    ctx.fp(g, dev(vol, tdt), after)
+    torch.cuda.synchronize()
+    assert torch.equal(before, after)
+    capi.fp_autotune_clear()

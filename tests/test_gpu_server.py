@@ -143,3 +143,20 @@ def test_served_bf16_storage_equals_alone(small_requests):
         print("bf16 working copies vs fp32, kind", r.kind, "rel-L2", err)
         assert err <= 2e-2
     assert np.array_equal(res[-1]["volume"], alone(extra))
+
+
+def test_served_autotuned_equals_alone(small_requests):
+    """With autotuning on, every OS-SART request tunes the forward projector
+    for its geometry first; the results are still those of the alone runs."""
+    print_case("server autotune", [(r.kind, r.geom.nx) for r in small_requests])
+    capi.fp_autotune_clear()
+    ref = [alone(r) for r in small_requests]
+    for r in small_requests:
+        r.volume = None
+    try:
+        with S.Server(lanes=2, autotune=True) as srv:
+            res, _ = S.run_stream(srv, small_requests)
+    finally:
+        capi.fp_autotune_clear()
+    for r, out, want in zip(small_requests, res, ref):
+        assert np.array_equal(out["volume"], want), (r.kind, r.geom.nx)

@@ -291,6 +291,22 @@ def test_fdk_reconstruct_4d_checks_inputs():
         G.fdk_reconstruct_4d(g, angles, offs, p[:5], [[0, 1]])
 
 
+def test_os_sart_autotuned_is_identical():
+    """Tuning the forward projector for the scan's geometry first changes
+    no bit of the reconstruction."""
+    g, angles, offs, vol, p = problem(noise=True, seed=8)
+    print("CASE os-sart autotune", flush=True)
+    capi.fp_autotune_clear()
+    x0 = G.os_sart(g, angles, offs, p, 4, 2, nonneg=True)
+    G.set_autotune(True)
+    try:
+        x1 = G.os_sart(g, angles, offs, p, 4, 2, nonneg=True)
+    finally:
+        G.set_autotune(False)
+        capi.fp_autotune_clear()
+    assert np.array_equal(x0, x1)
+
+
 def test_solver_errors():
     g, angles, offs, vol, p = problem(nviews=6)
     with pytest.raises(RuntimeError, match="subsets"):
