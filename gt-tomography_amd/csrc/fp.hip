@@ -247,11 +247,14 @@ template <typename T>
 struct PairWord;
 template <>
 struct PairWord<float> {
-  using type = float2;
-  __device__ static float2 load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    return bload2<float>(r, voff, soff);
+  using type = unsigned long long;   // both floats in one 64-bit word (kept in registers)
+  __device__ static unsigned long long load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+    return (unsigned long long)(unsigned)v[0] | ((unsigned long long)(unsigned)v[1] << 32);
   }
-  __device__ static float2 unpack(float2 w) { return w; }
+  __device__ static float2 unpack(unsigned long long w) {
+    return make_float2(__uint_as_float((unsigned)w), __uint_as_float((unsigned)(w >> 32)));
+  }
 };
 template <>
 struct PairWord<unsigned short> {
@@ -396,15 +399,81 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
         W L00[NCH], L10[NCH], L01[NCH], L11[NCH];
         const unsigned zoff = (unsigned)zb0 * esz;
         int cx = -8, cy = -8;   // no cell loaded yet
-        auto load_line = [&](W(&L)[NCH], unsigned B) {
+        auto load_line = [&](W(&L)[NCH], unsigned B) __attribute__((always_inline)) {
 #pragma unroll
           for (int c = 0; c < NCH; ++c) L[c] = PairWord<T>::load(rsrc, 2 * lane_off + (unsigned)(128 * c) * esz, B);
         };
-        auto copy_line = [&](W(&D)[NCH], const W(&S)[NCH]) {
+        auto copy_line = [&](W(&D)[NCH], const W(&S)[NCH]) __attribute__((always_inline)) {
 #pragma unroll
           for (int c = 0; c < NCH; ++c) D[c] = S[c];
         };
         const float *zb = zl - zb0;
+        // one step: the cell's corner lines (loaded or kept), the combined
+        // z-line into LDS, every ray's z-lerp from it
+        auto step = [&](const float fm, const float ax, const float ay, const int ix,
+                        const int iy) __attribute__((always_inline)) {
+            if (ix != cx || iy != cy) {
+              const int dx = ix - cx, dy = iy - cy;
+              const unsigned B = ((unsigned)iy * P.px + (unsigned)ix) * PZ * esz + zoff;
+              if (dy == 0 && dx == 1) {
+                copy_line(L00, L10); copy_line(L01, L11);
+                load_line(L10, B + so_x); load_line(L11, B + so_xy);
+              } else if (dy == 0 && dx == -1) {
+                copy_line(L10, L00); copy_line(L11, L01);
+                load_line(L00, B); load_line(L01, B + so_y);
+              } else if (dx == 0 && dy == 1) {
+                copy_line(L00, L01); copy_line(L10, L11);
+                load_line(L01, B + so_y); load_line(L11, B + so_xy);
+              } else if (dx == 0 && dy == -1) {
+                copy_line(L01, L00); copy_line(L11, L10);
+                load_line(L00, B); load_line(L10, B + so_x);
+              } else if (dx == 1 && dy == 1) {
+                copy_line(L00, L11);
+                load_line(L10, B + so_x); load_line(L01, B + so_y); load_line(L11, B + so_xy);
+              } else if (dx == 1 && dy == -1) {
+                copy_line(L01, L10);
+                load_line(L00, B); load_line(L10, B + so_x); load_line(L11, B + so_xy);
+              } else if (dx == -1 && dy == 1) {
+                copy_line(L10, L01);
+                load_line(L00, B); load_line(L01, B + so_y); load_line(L11, B + so_xy);
+              } else if (dx == -1 && dy == -1) {
+                copy_line(L11, L00);
+                load_line(L00, B); load_line(L10, B + so_x); load_line(L01, B + so_y);
+              } else {
+                load_line(L00, B); load_line(L10, B + so_x);
+                load_line(L01, B + so_y); load_line(L11, B + so_xy);
+              }
+              cx = ix;
+              cy = iy;
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+              const float2 v00 = PairWord<T>::unpack(L00[c]), v10 = PairWord<T>::unpack(L10[c]);
+              const float2 v01 = PairWord<T>::unpack(L01[c]), v11 = PairWord<T>::unpack(L11[c]);
+              float2 o;
+              {
+                const float c0 = fmaf(ax, v10.x - v00.x, v00.x);
+                const float c1 = fmaf(ax, v11.x - v01.x, v01.x);
+                o.x = fmaf(ay, c1 - c0, c0);
+              }
+              {
+                const float c0 = fmaf(ax, v10.y - v00.y, v00.y);
+                const float c1 = fmaf(ax, v11.y - v01.y, v01.y);
+                o.y = fmaf(ay, c1 - c0, c0);
+              }
+              *reinterpret_cast<float2 *>(&zl[2 * lane + 128 * c]) = o;
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+              const float tz = fmaf(fm, qz[r], pz[r]);  // in [0, nz+1]: checked above
+              const int iz = (int)tz;
+              const float az = __builtin_amdgcn_fractf(tz);
+              const float a = zb[iz], b = zb[iz + 1];
+              acc[r] += fmaf(az, b - a, a);
+            }
+            wave_lds_sync();
+        };
         for (int m = 0; m < P.n; ++m) {
           const float fm = (float)m;
           const float sx = __builtin_amdgcn_fmed3f(fmaf(fm, qx, px0), 0.0f, P.cx);
@@ -412,67 +481,7 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
           const float ax = __builtin_amdgcn_fractf(sx), ay = __builtin_amdgcn_fractf(sy);
           const int ix = __builtin_amdgcn_readfirstlane((int)sx);
           const int iy = __builtin_amdgcn_readfirstlane((int)sy);
-          if (ix != cx || iy != cy) {
-            const int dx = ix - cx, dy = iy - cy;
-            const unsigned B = ((unsigned)iy * P.px + (unsigned)ix) * PZ * esz + zoff;
-            if (dy == 0 && dx == 1) {
-              copy_line(L00, L10); copy_line(L01, L11);
-              load_line(L10, B + so_x); load_line(L11, B + so_xy);
-            } else if (dy == 0 && dx == -1) {
-              copy_line(L10, L00); copy_line(L11, L01);
-              load_line(L00, B); load_line(L01, B + so_y);
-            } else if (dx == 0 && dy == 1) {
-              copy_line(L00, L01); copy_line(L10, L11);
-              load_line(L01, B + so_y); load_line(L11, B + so_xy);
-            } else if (dx == 0 && dy == -1) {
-              copy_line(L01, L00); copy_line(L11, L10);
-              load_line(L00, B); load_line(L10, B + so_x);
-            } else if (dx == 1 && dy == 1) {
-              copy_line(L00, L11);
-              load_line(L10, B + so_x); load_line(L01, B + so_y); load_line(L11, B + so_xy);
-            } else if (dx == 1 && dy == -1) {
-              copy_line(L01, L10);
-              load_line(L00, B); load_line(L10, B + so_x); load_line(L11, B + so_xy);
-            } else if (dx == -1 && dy == 1) {
-              copy_line(L10, L01);
-              load_line(L00, B); load_line(L01, B + so_y); load_line(L11, B + so_xy);
-            } else if (dx == -1 && dy == -1) {
-              copy_line(L11, L00);
-              load_line(L00, B); load_line(L10, B + so_x); load_line(L01, B + so_y);
-            } else {
-              load_line(L00, B); load_line(L10, B + so_x);
-              load_line(L01, B + so_y); load_line(L11, B + so_xy);
-            }
-            cx = ix;
-            cy = iy;
-          }
-#pragma unroll
-          for (int c = 0; c < NCH; ++c) {
-            const float2 v00 = PairWord<T>::unpack(L00[c]), v10 = PairWord<T>::unpack(L10[c]);
-            const float2 v01 = PairWord<T>::unpack(L01[c]), v11 = PairWord<T>::unpack(L11[c]);
-            float2 o;
-            {
-              const float c0 = fmaf(ax, v10.x - v00.x, v00.x);
-              const float c1 = fmaf(ax, v11.x - v01.x, v01.x);
-              o.x = fmaf(ay, c1 - c0, c0);
-            }
-            {
-              const float c0 = fmaf(ax, v10.y - v00.y, v00.y);
-              const float c1 = fmaf(ax, v11.y - v01.y, v01.y);
-              o.y = fmaf(ay, c1 - c0, c0);
-            }
-            *reinterpret_cast<float2 *>(&zl[2 * lane + 128 * c]) = o;
-          }
-          wave_lds_sync();
-#pragma unroll
-          for (int r = 0; r < RPL; ++r) {
-            const float tz = fmaf(fm, qz[r], pz[r]);  // in [0, nz+1]: checked above
-            const int iz = (int)tz;
-            const float az = __builtin_amdgcn_fractf(tz);
-            const float a = zb[iz], b = zb[iz + 1];
-            acc[r] += fmaf(az, b - a, a);
-          }
-          wave_lds_sync();
+          step(fm, ax, ay, ix, iy);
         }
       }
     }
