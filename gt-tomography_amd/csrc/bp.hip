@@ -92,8 +92,14 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
   // band of detector rows per view -- at a time (measured faster than giving
   // each XCD its own z-slab).
   const unsigned L = blockIdx.x;
-  const int x = (int)(L % gx) * BP_TX + threadIdx.x;
-  const int y = (int)((L / gx) % gy) * BP_TY + threadIdx.y;
+  // a wave's 64 columns are an 8 x 8 quarter of the 16 x 16 tile: their
+  // projections span fewer detector cache lines at every view angle than a
+  // 16 x 4 strip (measured 10% faster for FP32 working copies, 2% for BF16)
+  const int tid = threadIdx.x + BP_TX * threadIdx.y, wv = tid >> 6, ln = tid & 63;
+  const int lx = (wv & 1) * 8 + (ln & 7);
+  const int ly = (wv >> 1) * 8 + (ln >> 3);
+  const int x = (int)(L % gx) * BP_TX + lx;
+  const int y = (int)((L / gx) % gy) * BP_TY + ly;
   const int z0 = (int)(L / (gx * gy)) * KZ;
   if (x >= P.nx || y >= P.ny) return;
   // voxel centre in mm (xray/conebeam_projection.cu:945-959)
@@ -293,7 +299,10 @@ BPParams make_params(const cbct_geom *g, int nviews) {
   return P;
 }
 
-constexpr int KZ = 8;
+// z-voxels per thread: the per-(x, y, view) work (rotation, magnification,
+// u coordinate and weight, FDK weight) is shared by KZ voxel-views (measured:
+// 16 beats 8 by 4-5% and 32 by a third at 512^3)
+constexpr int KZ = 16;
 
 template <typename T, bool QUAD>
 void launch_bp(const BPParams &P, dim3 grid, hipStream_t s, const ViewF *views, const T *pp,
