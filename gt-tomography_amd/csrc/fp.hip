@@ -395,13 +395,14 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
         // the lines as loaded: FP32 pairs, or BF16 pairs packed in one word
         // (unpacked in the combine, which keeps the registers, and with them
         // the occupancy, of the uncached variant)
-        using W = typename PairWord<T>::type;
+        using PW = PairWord<T>;
+        using W = typename PW::type;
         W L00[NCH], L10[NCH], L01[NCH], L11[NCH];
         const unsigned zoff = (unsigned)zb0 * esz;
         int cx = -8, cy = -8;   // no cell loaded yet
         auto load_line = [&](W(&L)[NCH], unsigned B) __attribute__((always_inline)) {
 #pragma unroll
-          for (int c = 0; c < NCH; ++c) L[c] = PairWord<T>::load(rsrc, 2 * lane_off + (unsigned)(128 * c) * esz, B);
+          for (int c = 0; c < NCH; ++c) L[c] = PW::load(rsrc, 2 * lane_off + (unsigned)(128 * c) * esz, B);
         };
         auto copy_line = [&](W(&D)[NCH], const W(&S)[NCH]) __attribute__((always_inline)) {
 #pragma unroll
