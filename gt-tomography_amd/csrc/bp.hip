@@ -32,6 +32,7 @@ struct BPParams {
   float cu, cv;        // padded texel offset: n/2 - 0.5 + 1
   float clu, clv;      // clamp limits (= n + 1)
   unsigned pnu;        // padded row length
+  unsigned pnv;        // padded rows
   unsigned pview;      // padded elements per view
 };
 
@@ -87,6 +88,9 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
                                                      const T *__restrict__ ppad,
                                                      float *__restrict__ vol, int accumulate,
                                                      unsigned gx, unsigned gy) {
+  // only the explicit fmaf calls fuse: the kernels' bits must not depend on
+  // how the compiler contracts what it simplifies (fmaf(1, a*b, c) = a*b + c)
+#pragma clang fp contract(off)
   // linear tile id, x fastest, z slowest.  Consecutive tiles land on
   // different XCDs, so all XCDs work on one band of the volume -- and one
   // band of detector rows per view -- at a time (measured faster than giving
@@ -152,6 +156,153 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
     }
     vw = nxt;
   }
+#pragma unroll
+  for (int t = 0; t < KZ; ++t) {
+    const int z = z0 + t;
+    if (z < P.nz) {
+      const size_t idx = (size_t)x + (size_t)P.nx * ((size_t)y + (size_t)P.ny * z);
+      vol[idx] = accumulate ? vol[idx] + acc[t] : acc[t];
+    }
+  }
+}
+
+// LDS-staged variant (FP32 working copies).  Per view, the block's detector
+// rectangle (from the projections of its four corner columns at its first
+// and last z, which bound every voxel of the block: u is a linear-fractional
+// function of (x, y) and v is linear in the magnification, so both are
+// extreme at the corners; one texel of margin covers rounding) is copied
+// from the plain zero-bordered copy into LDS by direct global-to-LDS loads,
+// one view ahead (the next view's rectangle streams in while this view is
+// computed), and every voxel takes its four taps from there.  Same
+// arithmetic, same bits as k_bp.  Rectangles wider than 128 texels or
+// taller than the buffer are read tap by tap.
+constexpr int BPS_CAP = 4096;   // floats per LDS buffer
+
+struct BPSRect {
+  int u0, v0, W, H, rw;   // origin, size, LDS row stride (64 or 128 floats)
+  bool staged;
+};
+
+template <bool FBP, int KZ>
+__global__ void __launch_bounds__(BP_TX *BP_TY) k_bps(BPParams P, const ViewF *__restrict__ views,
+                                                      const float *__restrict__ ppad,
+                                                      float *__restrict__ vol, int accumulate,
+                                                      unsigned gx, unsigned gy) {
+#pragma clang fp contract(off)
+  __shared__ float buf[2 * BPS_CAP];   // indexed directly: LDS instructions, not flat ones
+  const unsigned L = blockIdx.x;
+  const int tid = threadIdx.x + BP_TX * threadIdx.y, wv = tid >> 6, ln = tid & 63;
+  const int xb = (int)(L % gx) * BP_TX, yb = (int)((L / gx) % gy) * BP_TY;
+  const int x = xb + (wv & 1) * 8 + (ln & 7);
+  const int y = yb + (wv >> 1) * 8 + (ln >> 3);
+  const int z0 = (int)(L / (gx * gy)) * KZ;
+  const bool active = x < P.nx && y < P.ny;   // the others stage and synchronise only
+  const float px = ((x + 0.5f) / P.nx - 0.5f) * P.lx;
+  const float py = ((y + 0.5f) / P.ny - 0.5f) * P.ly;
+  const float dz = P.lz / P.nz;
+  const float pz0 = ((z0 + 0.5f) / P.nz - 0.5f) * P.lz;
+  // every lane evaluates the corner column ln & 3 of the block
+  const int cxi = xb + ((ln & 1) ? BP_TX - 1 : 0), cyi = yb + ((ln & 2) ? BP_TY - 1 : 0);
+  const float cpx = ((cxi + 0.5f) / P.nx - 0.5f) * P.lx;
+  const float cpy = ((cyi + 0.5f) / P.ny - 0.5f) * P.ly;
+  const int view_bytes = (int)(P.pview * sizeof(float));
+  auto rsrc_of = [&](int k) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(ppad + (size_t)k * P.pview), (short)0, view_bytes,
+                                             0x00020000);
+  };
+  auto rect_of = [&](int k) {
+    const ViewF vw = views[k];
+    const float cqx = fmaf(vw.c, cpx, vw.s * cpy);
+    const float cqy = fmaf(vw.c, cpy, -vw.s * cpx);
+    const float cmag = P.sdd * __builtin_amdgcn_rcpf(cqy + P.sad);
+    const float ctu = __builtin_amdgcn_fmed3f(fmaf(fmaf(cqx, cmag, -vw.off_u), P.su, P.cu), 0.0f, P.clu);
+    const float ctv0 = fmaf(fmaf(pz0, cmag, -vw.off_v), P.sv, P.cv);
+    const float ca = __builtin_amdgcn_fmed3f(ctv0, 0.0f, P.clv);
+    const float cb = __builtin_amdgcn_fmed3f(fmaf((float)(KZ - 1), dz * cmag * P.sv, ctv0), 0.0f, P.clv);
+    int ulo = (int)ctu, uhi = ulo, vlo = (int)fminf(ca, cb), vhi = (int)fmaxf(ca, cb);
+#pragma unroll
+    for (int o = 1; o <= 2; o <<= 1) {
+      ulo = min(ulo, __shfl_xor(ulo, o, 64));
+      uhi = max(uhi, __shfl_xor(uhi, o, 64));
+      vlo = min(vlo, __shfl_xor(vlo, o, 64));
+      vhi = max(vhi, __shfl_xor(vhi, o, 64));
+    }
+    BPSRect R;
+    R.u0 = __builtin_amdgcn_readfirstlane(max(ulo - 1, 0));
+    R.v0 = __builtin_amdgcn_readfirstlane(max(vlo - 1, 0));
+    R.W = __builtin_amdgcn_readfirstlane(min(uhi + 2, (int)P.pnu - 1)) - R.u0 + 1;
+    R.H = __builtin_amdgcn_readfirstlane(min(vhi + 2, (int)P.pnv - 1)) - R.v0 + 1;
+    R.rw = R.W <= 64 ? 64 : 128;
+    R.staged = R.W <= 128 && R.H * R.rw <= BPS_CAP;
+    return R;
+  };
+  // direct global-to-LDS copy of a rectangle (each instruction: 64 floats of one row)
+  auto issue = [&](int k, const BPSRect &R, int base) {
+    const __amdgpu_buffer_rsrc_t rs = rsrc_of(k);
+    for (int r = wv; r < R.H; r += 4)
+      for (int c0 = 0; c0 < R.W; c0 += 64)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rs, (__attribute__((address_space(3))) void *)&buf[base + r * R.rw + c0], 4,
+            (unsigned)(((R.v0 + r) * (int)P.pnu + R.u0 + c0 + ln) * 4), 0, 0, 0);
+  };
+  float acc[KZ];
+#pragma unroll
+  for (int t = 0; t < KZ; ++t) acc[t] = 0.0f;
+  BPSRect cur = rect_of(0);
+  if (cur.staged) issue(0, cur, 0);
+  for (int k = 0; k < P.nviews; ++k) {
+    const ViewF vw = views[k];
+    const __amdgpu_buffer_rsrc_t rsrc = rsrc_of(k);
+    const float qx = fmaf(vw.c, px, vw.s * py);
+    const float qy = fmaf(vw.c, py, -vw.s * px);
+    const float mag = P.sdd * __builtin_amdgcn_rcpf(qy + P.sad);
+    const float tu = __builtin_amdgcn_fmed3f(fmaf(fmaf(qx, mag, -vw.off_u), P.su, P.cu), 0.0f, P.clu);
+    const float au = __builtin_amdgcn_fractf(tu);
+    const int iu = (int)tu;
+    const float tv0 = fmaf(fmaf(pz0, mag, -vw.off_v), P.sv, P.cv);
+    const float dtv = dz * mag * P.sv;
+    float w = 1.0f;
+    if (FBP) {
+      const float U = (P.sad - qy) * P.inv_sad;
+      w = __builtin_amdgcn_rcpf(U * U);
+    }
+    // this wave's copies of view k have landed; after the barrier everyone's
+    // have, and nobody reads view k-1's buffer any more
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    BPSRect nxt{};
+    if (k + 1 < P.nviews) {
+      nxt = rect_of(k + 1);
+      if (nxt.staged) issue(k + 1, nxt, ((k + 1) & 1) * BPS_CAP);
+    }
+    const int base = (k & 1) * BPS_CAP;
+    const int du = iu - cur.u0;
+#pragma unroll
+    for (int t = 0; t < KZ; ++t) {
+      const float tv = __builtin_amdgcn_fmed3f(fmaf((float)t, dtv, tv0), 0.0f, P.clv);
+      const float av = __builtin_amdgcn_fractf(tv);
+      const int iv = (int)tv;
+      float a, bb, c, d;
+      if (cur.staged) {
+        const int q = base + (iv - cur.v0) * cur.rw + du;
+        a = buf[q];
+        bb = buf[q + 1];
+        c = buf[q + cur.rw];
+        d = buf[q + cur.rw + 1];
+      } else {
+        const unsigned o = ((unsigned)iv * P.pnu + (unsigned)iu) * 4u;
+        a = bld<float>(rsrc, o, 0);
+        bb = bld<float>(rsrc, o + 4, 0);
+        c = bld<float>(rsrc, o, P.pnu * 4);
+        d = bld<float>(rsrc, o + 4, P.pnu * 4);
+      }
+      const float r0 = fmaf(au, bb - a, a);
+      const float r1 = fmaf(au, d - c, c);
+      acc[t] = fmaf(w, fmaf(av, r1 - r0, r0), acc[t]);
+    }
+    cur = nxt;
+  }
+  if (!active) return;
 #pragma unroll
   for (int t = 0; t < KZ; ++t) {
     const int z = z0 + t;
@@ -235,6 +386,24 @@ __global__ void __launch_bounds__(256) k_pad_quad(const TI *__restrict__ in, T *
   }
 }
 
+// Plain zero-bordered copy with the storage conversion (FP32 -> BF16).
+template <typename TI, typename T>
+__global__ void __launch_bounds__(256) k_pad_plain(const TI *__restrict__ in, T *__restrict__ out,
+                                                   int nu, int nv, unsigned pnu, unsigned pnv,
+                                                   size_t total) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const unsigned a = (unsigned)(t % pnu);
+    const unsigned b = (unsigned)((t / pnu) % pnv);
+    const size_t k = t / ((size_t)pnu * pnv);
+    const int u = (int)a - 1, v = (int)b - 1;
+    T val = T(0);
+    if (u >= 0 && u < nu && v >= 0 && v < nv)
+      val = to_storage<T>(in[(size_t)u + (size_t)nu * ((size_t)v + (size_t)nv * k)]);
+    out[t] = val;
+  }
+}
+
 // Space carver (xray/conebeam_projection.cu:822-921): nearest-texel lookups,
 // z axis flipped, count of views with 0 != p < limit; mask = count < 3.
 __global__ void __launch_bounds__(256) k_spacecarver(BPParams P, int nu, int nv,
@@ -295,6 +464,7 @@ BPParams make_params(const cbct_geom *g, int nviews) {
   P.clu = (float)(g->nu + 1);
   P.clv = (float)(g->nv + 1);
   P.pnu = (unsigned)padded_dim(g->nu);
+  P.pnv = (unsigned)padded_dim(g->nv);
   P.pview = (unsigned)(padded_dim(g->nu) * padded_dim(g->nv));
   return P;
 }
@@ -322,7 +492,30 @@ void launch_bp(const BPParams &P, dim3 grid, hipStream_t s, const ViewF *views, 
 template <typename TI, typename T>
 int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int view0, int nviews,
            const TI *proj, float *vol, int fbp, int accumulate, hipStream_t s) {
-  // the plain layout converts nothing: BF16 working copies use quads
+  // engine 2: the LDS-staged kernel on a plain copy for FP32 working copies
+  // (BF16 ones use quads, measured faster for them); engine 0's plain layout
+  // converts nothing: BF16 working copies use quads
+  if (ctx->bp_engine == 2 && sizeof(T) == 4) {
+    const size_t totalp = (size_t)P.pview * nviews;
+    int rc2 = cbct_reserve(ctx, ctx->proj_pad, totalp * sizeof(T));
+    if (rc2) return rc2;
+    T *pp2 = static_cast<T *>(ctx->proj_pad.ptr);
+    hipLaunchKernelGGL((k_pad_plain<TI, T>), stride_grid(totalp), dim3(256), 0, s, proj, pp2, g->nu, g->nv,
+                       P.pnu, P.pnv, totalp);
+    CBCT_LAUNCH_CHECK(ctx, "k_pad_plain");
+    cbct_prof_mark(ctx, PROF_BP, 0, s);
+    const unsigned nblocks = grid.x * grid.y * grid.z;
+    const float *pf = reinterpret_cast<const float *>(pp2);
+    if (fbp)
+      hipLaunchKernelGGL((k_bps<true, KZ>), dim3(nblocks), dim3(BP_TX, BP_TY), 0, s, P,
+                         ctx->d_views_f + view0, pf, vol, accumulate, grid.x, grid.y);
+    else
+      hipLaunchKernelGGL((k_bps<false, KZ>), dim3(nblocks), dim3(BP_TX, BP_TY), 0, s, P,
+                         ctx->d_views_f + view0, pf, vol, accumulate, grid.x, grid.y);
+    cbct_prof_mark(ctx, PROF_BP, 1, s);
+    CBCT_LAUNCH_CHECK(ctx, "k_bps");
+    return CBCT_OK;
+  }
   const bool quad = ctx->bp_engine == 1 || sizeof(TI) != sizeof(T);
   const unsigned pnu = (unsigned)padded_dim(g->nu), pnv = (unsigned)padded_dim(g->nv);
   const size_t total = (size_t)P.pview * nviews;
@@ -401,7 +594,7 @@ int cbct_storage(cbct_ctx ctx, int dtype) {
 
 int cbct_bp_engine(cbct_ctx ctx, int engine) {
   if (!ctx) return CBCT_ERR_ARG;
-  if (engine != 0 && engine != 1)
+  if (engine < 0 || engine > 2)
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_bp_engine: engine %d not available", engine);
   ctx->bp_engine = engine;
   return CBCT_OK;

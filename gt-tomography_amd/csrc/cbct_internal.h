@@ -44,7 +44,7 @@ struct cbct_ctx_s {
   int filter_nu = 0;
   int filter_engine = 0;
   int fp_engine = 1;             // 1: detector-column sharing, 0: one thread per ray
-  int bp_engine = 1;             // 1: quad-texel projection copy, 0: plain padded copy
+  int bp_engine = 2;             // 2: LDS-staged (FP32 copies) / quad (BF16), 1: quad, 0: plain
   int fp_max_rpl = 8;            // column engine: most detector rows per lane
   int fp_force_variant = 0;      // autotuner trials: variant and tile forced when > 0
   int fp_force_rpl = 0;

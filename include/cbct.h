@@ -133,11 +133,15 @@ const char *cbct_fp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nvie
  * reference counterpart. */
 int cbct_fp_tile(cbct_ctx ctx, int max_rows_per_lane);
 
-/* Backprojector engine: 1 = quad-texel copy of the projections (default; the
- * four bilinear taps of a voxel-view in one 16-byte (FP32) or 8-byte (BF16)
- * load, 4x the padded projection workspace), 0 = plain zero-bordered copy.
- * Both give bitwise-identical volumes.  No reference counterpart (the
- * reference samples a layered texture). */
+/* Backprojector engine: 2 = default: with FP32 working copies, the
+ * LDS-staged kernel (per view, each block's detector rectangle is copied from
+ * a plain zero-bordered copy into LDS by direct global-to-LDS loads, one view
+ * ahead, and read from there), with BF16 working copies as 1; 1 = quad-texel
+ * copy of the projections (the four bilinear taps of a voxel-view in one
+ * 16-byte (FP32) or 8-byte (BF16) load, 4x the padded projection workspace);
+ * 0 = plain zero-bordered copy, taps loaded one by one.  All give
+ * bitwise-identical volumes.  No reference counterpart (the reference samples
+ * a layered texture). */
 int cbct_bp_engine(cbct_ctx ctx, int engine);
 
 /* Storage type of the projectors' working copies (the zero-bordered volume

@@ -135,8 +135,9 @@ def test_bp_parity(ctx, name, fbp):
 @pytest.mark.parametrize("dtype", ["f32", "bf16"])
 @pytest.mark.parametrize("name", list(CASES))
 def test_bp_engines_bitwise_identical(ctx, name, dtype):
-    """Quad-texel and plain padded projection copies give identical bits
-    (FBP and plain, FP32 and BF16 storage, accumulating into a volume)."""
+    """Quad-texel and plain padded projection copies, and the LDS-staged
+    kernel, give identical bits (FBP and plain, FP32 and BF16 storage,
+    accumulating into a volume)."""
     g, angles, offsets = make_case(name)
     print_case("bp engines", name, dtype, g)
     proj = np.random.default_rng(4).random(g.proj_shape(len(angles)))
@@ -145,14 +146,15 @@ def test_bp_engines_bitwise_identical(ctx, name, dtype):
     start = torch.from_numpy(np.random.default_rng(6).random(g.vol_shape).astype(np.float32)).cuda()
     for fbp in (False, True):
         outs = []
-        for e in (0, 1):
+        for e in (0, 1, 2):
             ctx.bp_engine(e)
             out = start.clone()
             ctx.bp(g, dev(proj, tdt), out, fbp=fbp, accumulate=True)
             torch.cuda.synchronize()
             outs.append(out)
-        ctx.bp_engine(1)
+        ctx.bp_engine(2)
         assert torch.equal(outs[0], outs[1]), fbp
+        assert torch.equal(outs[0], outs[2]), fbp
 
 
 @pytest.mark.parametrize("name", list(CASES))
