@@ -667,7 +667,10 @@ def fdk_mult_MH(geom: Geom, angles, offsets, proj, binning: Binning, F, short_sc
             filt = fdk_prefilter(geom, angles[bin_], offsets[bin_], pb, F, short_scan,
                                  use_offset_correction, cos_w=cw)
             bp(geom, angles[bin_], offsets[bin_], filt, fbp=True, out=view, accumulate=accumulate)
-        view *= np.float32(2.0) / np.float32(len(bin_)) if len(bin_) else np.inf
+        # the reference scales by 2/|bin| unconditionally: an empty bin's zero
+        # volume becomes 0 * inf = NaN there, and here
+        with np.errstate(invalid="ignore"):
+            view *= np.float32(2.0) / np.float32(len(bin_)) if len(bin_) else np.inf
         of[b] = view.reshape(-1)
     return out
 
