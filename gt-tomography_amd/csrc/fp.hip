@@ -390,7 +390,8 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
       const int zhi_max = __builtin_amdgcn_readfirstlane((int)fmaxf(pzh, fmaf(ml, qzh, pzh))) + 1;
       const int zb0 = zlo_min & ~1;
       const int nchw = (zhi_max - zb0 + 128) >> 7;
-      if (nchw <= NCH) {
+      // (cell coordinates are packed as two 16-bit halves below)
+      if (nchw <= NCH && P.cx < 65535.0f && P.cy < 65535.0f) {
         done = true;
         // the lines as loaded: FP32 pairs, or BF16 pairs packed in one word
         // (unpacked in the combine, which keeps the registers, and with them
@@ -475,7 +476,27 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
             }
             wave_lds_sync();
         };
-        for (int m = 0; m < P.n; ++m) {
+        // The x/y part of a step is the same for the whole wave.  BF16
+        // copies: it is evaluated for 64 steps at once, step mb + lane in
+        // each lane (the same expressions as per step, so the same values),
+        // and each step reads its cell and weights back into scalar
+        // registers.  FP32 copies: per step (measured faster there).
+        for (int mb = 0; sizeof(T) == 2 && mb < P.n; mb += 64) {
+          const float fml = (float)(mb + lane);
+          const float sxl = __builtin_amdgcn_fmed3f(fmaf(fml, qx, px0), 0.0f, P.cx);
+          const float syl = __builtin_amdgcn_fmed3f(fmaf(fml, qy, py0), 0.0f, P.cy);
+          const int axl = __float_as_int(__builtin_amdgcn_fractf(sxl));
+          const int ayl = __float_as_int(__builtin_amdgcn_fractf(syl));
+          const int cl = (int)((unsigned)sxl | ((unsigned)syl << 16));
+          const int cnt = __builtin_amdgcn_readfirstlane(min(64, P.n - mb));
+          for (int l = 0; l < cnt; ++l) {
+            const unsigned c = (unsigned)__builtin_amdgcn_readlane(cl, l);
+            const float ax = __int_as_float(__builtin_amdgcn_readlane(axl, l));
+            const float ay = __int_as_float(__builtin_amdgcn_readlane(ayl, l));
+            step((float)(mb + l), ax, ay, (int)(c & 0xffffu), (int)(c >> 16));
+          }
+        }
+        for (int m = 0; sizeof(T) != 2 && m < P.n; ++m) {
           const float fm = (float)m;
           const float sx = __builtin_amdgcn_fmed3f(fmaf(fm, qx, px0), 0.0f, P.cx);
           const float sy = __builtin_amdgcn_fmed3f(fmaf(fm, qy, py0), 0.0f, P.cy);
