@@ -46,8 +46,10 @@ struct cbct_ctx_s {
   int fp_engine = 1;             // 1: detector-column sharing, 0: one thread per ray
   int bp_engine = 2;             // 2: LDS-staged (FP32 copies) / quad (BF16), 1: quad, 0: plain
   int fp_max_rpl = 8;            // column engine: most detector rows per lane
+  int fp_view_group = 6;         // column engine: adjacent views launched together per column
   int fp_force_variant = 0;      // autotuner trials: variant and tile forced when > 0
   int fp_force_rpl = 0;
+  int fp_force_vg = 0;
   int work_dtype = CBCT_F32;     // type of the projectors' working copies of FP32 arrays
   float *d_h = nullptr;          // fp32 spatial kernel, 2*nu entries (h[|m|] mirrored)
   std::vector<double> h64;       // the same kernel in FP64, h[0..nu-1]

@@ -52,6 +52,7 @@ struct FPParams {
   double lx, ly, lz;
   double fu, fv, sad, sdd;
   float cx, cy, cz;           // clamp limits in padded texel space (= n_axis + 1)
+  unsigned vg;                // column engine: views per launch group (see col_block)
 };
 
 template <typename T>
@@ -275,6 +276,29 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Block order of the column engine.  xcd_remap gives every XCD a contiguous
+// range of W.  Within it, the blocks of P.vg adjacent views of one (column,
+// row group) come one after another, so they run at the same time on one XCD
+// and their rays, close together in the volume, share that XCD's L2; views
+// past the last whole group of P.vg follow view by view.  vg = 1: view by
+// view (column fastest).
+__device__ __forceinline__ void col_block(const FPParams &P, unsigned W, unsigned ngroups,
+                                          unsigned nblocks, int &i, int &grp, int &k) {
+  const unsigned per_view = (unsigned)P.nu * ngroups;
+  const unsigned nviews = nblocks / per_view;
+  const unsigned vg = P.vg;
+  const unsigned whole = (nviews / vg) * vg * per_view;
+  unsigned vin = 0, rest = W;
+  if (W < whole) {
+    vin = W % vg;
+    rest = W / vg;
+  }
+  i = (int)(rest % (unsigned)P.nu);
+  const unsigned r2 = rest / (unsigned)P.nu;
+  grp = (int)(r2 % ngroups);
+  k = W < whole ? (int)((r2 / ngroups) * vg + vin) : (int)(r2 / ngroups);
+}
+
 // One 64-lane workgroup per (detector column, row group, view).  RPL rays per
 // lane: rows j = row0 + r*64 + lane.
 //   SSTEP = false, ZP = 1: per-step setup on the vector unit, one z per lane
@@ -300,11 +324,8 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
                                             int accumulate, unsigned ngroups, unsigned nblocks,
                                             unsigned vt_bytes) {
   extern __shared__ float zl[];  // one interpolated z-line (whole 128-wide chunks)
-  const unsigned W = xcd_remap(blockIdx.x, nblocks);
-  const int i = (int)(W % (unsigned)P.nu);
-  const unsigned rest = W / (unsigned)P.nu;
-  const int grp = (int)(rest % ngroups);
-  const int k = (int)(rest / ngroups);
+  int i, grp, k;
+  col_block(P, xcd_remap(blockIdx.x, nblocks), ngroups, nblocks, i, grp, k);
   const int lane = threadIdx.x;
   const ViewD vw = views[k];
   const ColSetup C = col_setup(P, vw, i);
@@ -804,6 +825,7 @@ int make_params(cbct_ctx ctx, const cbct_geom *g, float spp, FPParams &P) {
   P.cx = (float)(g->nx + 1);
   P.cy = (float)(g->ny + 1);
   P.cz = (float)(g->nz + 1);
+  P.vg = (unsigned)ctx->fp_view_group;
   return CBCT_OK;
 }
 
@@ -879,6 +901,7 @@ void dispatch_col(const FPParams &P, int variant, int rpl, hipStream_t s, const 
 struct FPChoice {
   int variant;
   int rpl;
+  int vg = 0;   // views per launch group; 0 = the context's setting
 };
 
 double window_bound(const cbct_geom *g, int nv, int rpl, float omin, float omax) {
@@ -947,7 +970,7 @@ FPChoice select_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int
   const int rpl = rows_per_lane(P.nv, ctx->fp_max_rpl);
   if (ctx->fp_engine == 0 || vt_bytes + (size_t)P.pzx * esz * 2 >= (1ull << 31) || zline_bytes > 65536)
     return {0, rpl};
-  if (ctx->fp_force_variant > 0) return {ctx->fp_force_variant, ctx->fp_force_rpl};
+  if (ctx->fp_force_variant > 0) return {ctx->fp_force_variant, ctx->fp_force_rpl, ctx->fp_force_vg};
   const int per_step = esz == 2 ? 2 : 1;
   if (ctx->fp_engine == 2) return {3 - per_step, rpl};
   if (ctx->fp_engine >= 3) return {ctx->fp_engine, rpl};
@@ -994,8 +1017,10 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
   hipLaunchKernelGGL((k_pad_transpose<TI, T>), tgrid, dim3(32, 8), 0, s, vol, vt, g->nx, g->ny, g->nz,
                      P.px, P.pz);
   CBCT_LAUNCH_CHECK(ctx, "k_pad_transpose");
+  FPParams Q = P;
+  if (choice.vg > 0) Q.vg = (unsigned)choice.vg;
   cbct_prof_mark(ctx, PROF_FP, 0, s);
-  dispatch_col<T>(P, choice.variant, choice.rpl, s, views, vt, proj, accumulate, nviews,
+  dispatch_col<T>(Q, choice.variant, choice.rpl, s, views, vt, proj, accumulate, nviews,
                   (unsigned)vt_bytes);
   cbct_prof_mark(ctx, PROF_FP, 1, s);
   CBCT_LAUNCH_CHECK(ctx, "k_fp_col");
@@ -1076,9 +1101,10 @@ int cbct_fp_autotune(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, in
   hipLaunchKernelGGL(k_zero, stride_grid(vol_words), dim3(256), 0, s, vol_words, static_cast<float *>(vol));
   FPChoice best = heuristic;
   double best_t = 1e300;
-  for (size_t c = 0; c < cand.size() && rc == CBCT_OK; ++c) {
-    ctx->fp_force_variant = cand[c].variant;
-    ctx->fp_force_rpl = cand[c].rpl;
+  auto trial = [&](const FPChoice &c) {
+    ctx->fp_force_variant = c.variant;
+    ctx->fp_force_rpl = c.rpl;
+    ctx->fp_force_vg = c.vg;
     double t = 1e300;
     for (int rep = 0; rep < 3 && rc == CBCT_OK; ++rep) {   // first run warms up
       (void)hipEventRecord(e0, s);
@@ -1091,11 +1117,18 @@ int cbct_fp_autotune(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, in
     }
     if (t < best_t) {
       best_t = t;
-      best = cand[c];
+      best = c;
     }
-  }
+  };
+  // the variants and tiles at the context's launch order, then the launch
+  // order (views per group) for the fastest of them
+  for (size_t c = 0; c < cand.size() && rc == CBCT_OK; ++c) trial(cand[c]);
+  const FPChoice base = best;
+  for (int vg : {1, 3, 6, 8})
+    if (rc == CBCT_OK && vg != ctx->fp_view_group) trial({base.variant, base.rpl, vg});
   ctx->fp_force_variant = 0;
   ctx->fp_force_rpl = 0;
+  ctx->fp_force_vg = 0;
   (void)hipStreamSynchronize(s);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1121,6 +1154,14 @@ int cbct_fp_tile(cbct_ctx ctx, int max_rows_per_lane) {
   if (max_rows_per_lane < 1 || max_rows_per_lane > 12)
     return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_fp_tile: rows per lane %d outside 1..12", max_rows_per_lane);
   ctx->fp_max_rpl = max_rows_per_lane;
+  return CBCT_OK;
+}
+
+int cbct_fp_view_group(cbct_ctx ctx, int views) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (views < 1 || views > 64)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_fp_view_group: %d views outside 1..64", views);
+  ctx->fp_view_group = views;
   return CBCT_OK;
 }
 

@@ -106,6 +106,7 @@ def load():
         "cbct_fp_autotune": (I, [P, G, I, I, I, F, I, P, ctypes.POINTER(ctypes.c_double)]),
         "cbct_fp_autotune_clear": (I, []),
         "cbct_fp_tile": (I, [P, I]),
+        "cbct_fp_view_group": (I, [P, I]),
         "cbct_bp_engine": (I, [P, I]),
         "cbct_storage": (I, [P, I]),
         "cbct_spacecarver": (I, [P, G, I, I, P, P, F, P]),
@@ -276,8 +277,13 @@ class Context:
         """Column engine: most detector rows per lane (1..12)."""
         self._check(self.lib.cbct_fp_tile(self.h, int(max_rows_per_lane)))
 
+    def fp_view_group(self, views):
+        """Column engine: adjacent views launched together per detector column (1..64)."""
+        self._check(self.lib.cbct_fp_view_group(self.h, int(views)))
+
     def bp_engine(self, engine):
-        """1 = quad-texel projection copy (default), 0 = plain padded copy."""
+        """2 = default (LDS-staged kernel for FP32 copies, else as 1), 1 = quad-texel
+        projection copy, 0 = plain padded copy."""
         self._check(self.lib.cbct_bp_engine(self.h, int(engine)))
 
     def storage(self, dtype):

@@ -133,6 +133,14 @@ const char *cbct_fp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nvie
  * reference counterpart. */
 int cbct_fp_tile(cbct_ctx ctx, int max_rows_per_lane);
 
+/* Column-engine launch order: the blocks of `views` (1..64) adjacent views
+ * of one detector column and row group are launched one after another, so
+ * that they run together on one XCD and share its L2 (their rays pass close
+ * to each other through the volume).  1 = view by view; default 6.
+ * cbct_fp_autotune also tries 1, 3, 6 and 8.  Results do not depend on it.
+ * No reference counterpart. */
+int cbct_fp_view_group(cbct_ctx ctx, int views);
+
 /* Backprojector engine: 2 = default: with FP32 working copies, the
  * LDS-staged kernel (per view, each block's detector rectangle is copied from
  * a plain zero-bordered copy into LDS by direct global-to-LDS loads, one view

@@ -117,6 +117,43 @@ def test_fp_engines_bitwise_identical(ctx, name, dtype):
         assert rel_l2(host(outs[1]), O.fp(to_oracle(g), angles, offsets, vol)) <= TOL
 
 
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_fp_view_groups_bitwise_identical(ctx, dtype):
+    """The column engine's launch order (adjacent views of one column
+    launched together, cbct_fp_view_group) changes no bit: 13 views (a
+    ragged last group for every group size but 1 and 13, and a group larger
+    than the view count), two row groups per column, a view offset."""
+    g = geom(16, 16, 12, 24, 150)
+    angles = angles_full(13)
+    print_case("view groups", dtype, g)
+    vol = phantom_volume(g) + 0.01
+    tdt = torch.float32 if dtype == "f32" else torch.bfloat16
+    ctx.set_views(angles, None)
+    ctx.fp_tile(2)   # 150 rows at 2 rows per lane: two row groups
+    ref = {}
+    try:
+        for e in (1, 2, 3):
+            for vg in (1, 2, 5, 6, 13, 64):
+                ctx.fp_engine(e)
+                ctx.fp_view_group(vg)
+                for view0, nv in ((0, 13), (2, 11)):
+                    out = torch.zeros(g.proj_shape(nv), device="cuda")
+                    ctx.fp(g, dev(vol, tdt), out, view0=view0, nviews=nv)
+                    torch.cuda.synchronize()
+                    key = (view0, nv)
+                    if key not in ref:
+                        ref[key] = out
+                    assert torch.equal(out, ref[key]), (e, vg, view0)
+    finally:
+        ctx.fp_engine(1)
+        ctx.fp_tile(8)
+        ctx.fp_view_group(6)
+    assert torch.equal(ref[(2, 11)], ref[(0, 13)][2:])
+    for bad in (0, 65):
+        with pytest.raises(capi.CbctError, match="views outside"):
+            ctx.fp_view_group(bad)
+
+
 @pytest.mark.parametrize("fbp", [False, True])
 @pytest.mark.parametrize("name", list(CASES))
 def test_bp_parity(ctx, name, fbp):
