@@ -310,13 +310,14 @@ __device__ __forceinline__ void col_block(const FPParams &P, unsigned W, unsigne
 //   NCH > 0 (with SSTEP, ZP = 2): the z window of the whole ray (all its
 //     steps, all shared rows) is fixed once; when it spans at most NCH
 //     chunks of 128 z, the four corner z-lines of the current (x, y) cell
-//     stay in registers, and a step into a neighbouring cell keeps the lines
-//     the two cells share (two of four for a move along x or y, one for a
-//     diagonal move) and loads only the others.  The column's (x, y) is
-//     wave-uniform, so the case split is a scalar branch.
-//     The lines are kept as (00, 10, 01, 11) of the cell and shifted by
-//     register moves.  Every line load fetches all NCH chunks (no per-chunk
-//     branches); chunks past the window are never read.
+//     stay in registers while the ray stays in the cell.  FP32 copies: a
+//     step into a neighbouring cell keeps the lines the two cells share (two
+//     of four for a move along x or y, one for a diagonal move) and loads
+//     only the others; the column's (x, y) is wave-uniform, so the case
+//     split is a scalar branch, and the lines are kept as (00, 10, 01, 11)
+//     of the cell and shifted by register moves.  BF16 copies: a cell change
+//     reloads all four.  Every line load fetches all NCH chunks (no
+//     per-chunk branches); chunks past the window are never read.
 // All produce the same bits as the per-ray engine.
 template <typename T, int RPL, bool SSTEP, int ZP, int NCH>
 __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__restrict__ views,
@@ -432,10 +433,20 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
         };
         const float *zb = zl - zb0;
         // one step: the cell's corner lines (loaded or kept), the combined
-        // z-line into LDS, every ray's z-lerp from it
+        // z-line into LDS, every ray's z-lerp from it.  BF16 copies reload all
+        // four lines on a cell change, straight into their registers (fewer
+        // instructions than keeping the shared lines, whose register moves
+        // wait on the new loads); FP32 copies, bound by their line bytes,
+        // load only the lines the new cell does not share.
         auto step = [&](const float fm, const float ax, const float ay, const int ix,
                         const int iy) __attribute__((always_inline)) {
-            if (ix != cx || iy != cy) {
+            if (sizeof(T) == 2 && (ix != cx || iy != cy)) {
+              const unsigned B = ((unsigned)iy * P.px + (unsigned)ix) * PZ * esz + zoff;
+              load_line(L00, B); load_line(L10, B + so_x);
+              load_line(L01, B + so_y); load_line(L11, B + so_xy);
+              cx = ix;
+              cy = iy;
+            } else if (ix != cx || iy != cy) {
               const int dx = ix - cx, dy = iy - cy;
               const unsigned B = ((unsigned)iy * P.px + (unsigned)ix) * PZ * esz + zoff;
               if (dy == 0 && dx == 1) {
