@@ -36,6 +36,10 @@ def load():
     L.gtt_set_default_storage.restype = I
     L.gtt_set_autotune.argtypes = [I]
     L.gtt_set_autotune.restype = I
+    L.gtt_set_graphs.argtypes = [I]
+    L.gtt_set_graphs.restype = I
+    L.gtt_graph_launches.argtypes = []
+    L.gtt_graph_launches.restype = ctypes.c_longlong
     L.gtt_fdk_reconstruct_3d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, D]
     L.gtt_fdk_reconstruct_4d.argtypes = [G, P, P, I, P, P, I, P, P, F, P, P, D]
     L.gtt_os_sart.argtypes = [G, P, P, I, P, P, I, P, I, I, F, F, I, P, D]
@@ -68,6 +72,17 @@ def set_autotune(on):
     """OS reconstructions this thread starts from now on tune the forward
     projector for their geometry first."""
     _check(load().gtt_set_autotune(int(bool(on))))
+
+
+def set_graphs(on):
+    """OS-SART reconstructions this thread starts from now on replay their
+    iterations 2.. as captured HIP graphs (gamma = 0 only; same bits)."""
+    _check(load().gtt_set_graphs(int(bool(on))))
+
+
+def graph_launches():
+    """HIP graphs this thread has launched so far."""
+    return int(load().gtt_graph_launches())
 
 
 def _check(rc):

@@ -93,6 +93,13 @@ int gtt_set_autotune(int on) {
   return 0;
 }
 
+int gtt_set_graphs(int on) {
+  set_default_graphs(on != 0);
+  return 0;
+}
+
+long long gtt_graph_launches(void) { return graph_launches(); }
+
 int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles, const float *offsets, int nviews,
                            const unsigned *bin_index, const int *bin_sizes, int nbins,
                            const float *projections, const float *filter, float spp,
@@ -154,6 +161,7 @@ int gtt_os_sart_weighted(const cbct_geom *g, const float *angles, const float *o
     solver.set_beta(beta);
     solver.set_gamma(gamma);
     solver.set_non_negativity_constraint(nonneg != 0);
+    solver.set_graphs(default_graphs());
     if (objective_out)
       solver.set_objective_callback([&](int it, double v) { objective_out[it] = v; });
     auto x = solver.solve(&b);

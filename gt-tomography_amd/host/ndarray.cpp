@@ -5,6 +5,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <map>
+#include <set>
 #include <vector>
 #include <string>
 
@@ -18,6 +19,8 @@ thread_local std::map<int, cbct_ctx> t_ctx;
 thread_local hipStream_t t_stream = nullptr;
 thread_local int t_storage = CBCT_F32;
 thread_local bool t_autotune = false;
+thread_local bool t_graphs = false;
+thread_local long long t_graph_launches = 0;
 
 void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -41,6 +44,9 @@ void set_default_storage(int dtype) { t_storage = dtype == CBCT_BF16 ? CBCT_BF16
 int default_storage() { return t_storage; }
 void set_default_autotune(bool on) { t_autotune = on; }
 bool default_autotune() { return t_autotune; }
+void set_default_graphs(bool on) { t_graphs = on; }
+bool default_graphs() { return t_graphs; }
+long long graph_launches() { return t_graph_launches; }
 void set_current_stream(void *stream) { t_stream = static_cast<hipStream_t>(stream); }
 void release_thread_resources() {
   trim_device_cache();
@@ -68,6 +74,8 @@ thread_local std::vector<CachedBlock> t_free_blocks;
 thread_local std::map<void *, size_t> t_block_sizes;   // live blocks from device_malloc
 thread_local size_t t_cached_bytes = 0;
 constexpr size_t kCacheLimit = size_t(32) << 30;
+// While a StreamGraph captures: the blocks freed into the cache meanwhile
+thread_local std::vector<void *> *t_capture_freed = nullptr;
 }  // namespace
 
 void trim_device_cache() {
@@ -108,12 +116,91 @@ void device_free(void *p) {
   const size_t bytes = it->second;
   t_block_sizes.erase(it);
   int dev = 0;
-  if (t_cached_bytes + bytes > kCacheLimit || hipGetDevice(&dev) != hipSuccess) {
+  const bool dev_ok = hipGetDevice(&dev) == hipSuccess;
+  if (t_capture_freed) {   // no hipFree while capturing: it synchronises
+    t_capture_freed->push_back(p);
+  } else if (t_cached_bytes + bytes > kCacheLimit || !dev_ok) {
     (void)hipFree(p);
     return;
   }
   t_free_blocks.push_back({p, bytes, dev, t_stream});
   t_cached_bytes += bytes;
+}
+
+StreamGraph::StreamGraph(const std::function<void()> &work) {
+  hipStream_t s = t_stream;
+  if (!s) throw std::runtime_error("StreamGraph: the default stream cannot be captured");
+  std::vector<void *> freed;
+  t_capture_freed = &freed;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    t_capture_freed = nullptr;
+    hip_check(e, "hipStreamBeginCapture");
+  }
+  hipGraph_t g = nullptr;
+  try {
+    work();
+  } catch (...) {
+    t_capture_freed = nullptr;
+    if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    throw;
+  }
+  t_capture_freed = nullptr;
+  hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    hip_check(e, "hipGraphInstantiate");
+  }
+  graph_ = g;
+  exec_ = x;
+  stream_ = s;
+  // reserve the blocks the captured work uses as temporaries
+  const std::set<void *> mine(freed.begin(), freed.end());
+  for (auto it = t_free_blocks.begin(); it != t_free_blocks.end();) {
+    if (mine.count(it->p)) {
+      held_.push_back({it->p, it->bytes, it->device, it->stream});
+      t_cached_bytes -= it->bytes;
+      it = t_free_blocks.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+StreamGraph::~StreamGraph() {
+  if (exec_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec_));
+  if (graph_) (void)hipGraphDestroy(static_cast<hipGraph_t>(graph_));
+  // back into the cache: reused only on the same stream, after the replays
+  for (const Block &b : held_) {
+    t_free_blocks.push_back({b.p, b.bytes, b.device, static_cast<hipStream_t>(b.stream)});
+    t_cached_bytes += b.bytes;
+  }
+}
+
+void StreamGraph::launch() {
+  hip_check(hipGraphLaunch(static_cast<hipGraphExec_t>(exec_), static_cast<hipStream_t>(stream_)),
+            "hipGraphLaunch");
+  ++t_graph_launches;
+}
+
+PrivateStream::PrivateStream(bool needed) {
+  if (!needed) return;
+  device_synchronize();
+  hipStream_t s = nullptr;
+  hip_check(hipStreamCreate(&s), "hipStreamCreate");
+  stream_ = s;
+  previous_ = t_stream;
+  t_stream = s;
+}
+
+PrivateStream::~PrivateStream() {
+  if (!stream_) return;
+  (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
+  t_stream = static_cast<hipStream_t>(previous_);
+  (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
 }
 void copy_d2d(void *dst, const void *src, size_t bytes) {
   hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, t_stream), "hipMemcpyAsync");

@@ -9,6 +9,7 @@
 
 #include <cstddef>
 #include <algorithm>
+#include <functional>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -37,6 +38,12 @@ int default_storage();
 // projector for their geometry first (cbct_fp_autotune; default off).
 void set_default_autotune(bool on);
 bool default_autotune();
+// Whether OS-SART reconstructions the calling thread starts replay their
+// iterations as captured HIP graphs (default off).
+void set_default_graphs(bool on);
+bool default_graphs();
+// Graphs the calling thread has launched (StreamGraph::launch) so far.
+long long graph_launches();
 // HIP device current on the calling thread.
 int current_device();
 // Throw std::runtime_error with the library's message when rc != 0.
@@ -47,6 +54,48 @@ void device_free(void *p);
 // Frees the device blocks the calling thread keeps for reuse (device_free
 // caches blocks per thread, device and stream).
 void trim_device_cache();
+// Executable HIP graph of the device work a function queues on the calling
+// thread's current stream (which must not be the default stream: it cannot
+// be captured).  The function runs once, during the capture, queueing
+// nothing; launch() replays its work on that stream.  Device blocks freed
+// into the thread's cache while capturing stay reserved for the graph until
+// it is destroyed, so no allocation outside it can share them between
+// replays; no block is returned to the device while capturing.
+class StreamGraph {
+ public:
+  explicit StreamGraph(const std::function<void()> &work);
+  ~StreamGraph();
+  StreamGraph(const StreamGraph &) = delete;
+  StreamGraph &operator=(const StreamGraph &) = delete;
+  void launch();
+
+  struct Block {
+    void *p;
+    size_t bytes;
+    int device;
+    void *stream;
+  };
+
+ private:
+  void *graph_ = nullptr, *exec_ = nullptr, *stream_ = nullptr;
+  std::vector<Block> held_;
+};
+
+// When `needed`, makes a new stream the calling thread's current stream for
+// the lifetime of the object (after the device has finished its queued
+// work); on destruction, waits for that stream, restores the previous one
+// and destroys it.
+class PrivateStream {
+ public:
+  explicit PrivateStream(bool needed);
+  ~PrivateStream();
+  PrivateStream(const PrivateStream &) = delete;
+  PrivateStream &operator=(const PrivateStream &) = delete;
+
+ private:
+  void *stream_ = nullptr, *previous_ = nullptr;
+};
+
 void copy_d2d(void *dst, const void *src, size_t bytes);
 void copy_h2d(void *dst, const void *src, size_t bytes);
 void copy_d2h(void *dst, const void *src, size_t bytes);

@@ -46,10 +46,19 @@ class osSARTSolver : public solverBase<ARRAY> {
   // after every iteration (:110-120); both are optional here.
   void set_objective_callback(std::function<void(int, double)> cb) { objective_cb_ = std::move(cb); }
   void set_dump(bool d) { dump_ = d; }
+  // Replay iterations 2.. as captured HIP graphs (one per subset order; the
+  // first iteration runs directly and sizes every workspace).  Applies when
+  // gamma = 0 (the step size is then the same every iteration) and the
+  // output is not verbose (that reads a norm back per subset); runs on a
+  // stream of its own when the thread has none.  Same results bit for bit.
+  void set_graphs(bool g) { graphs_ = g; }
   void set_encoding_operator(std::shared_ptr<subsetOperator<ARRAY>> op) { encoding_operator_ = std::move(op); }
 
   std::shared_ptr<ARRAY> solve(ARRAY *in) {
     if (!encoding_operator_) throw std::runtime_error("Error: cgSolver::compute_rhs : no encoding operator is set");
+    const bool graphs = graphs_ && gamma_ == 0.0f && iterations_ > 1 &&
+                        this->output_mode_ < solverBase<ARRAY>::OUTPUT_VERBOSE;
+    PrivateStream own_stream(graphs && current_stream() == nullptr);
     auto image_dims = encoding_operator_->get_domain_dimensions();
     if (image_dims->empty())
       throw std::runtime_error("Error: cgSolver::compute_rhs : encoding operator has not set domain dimension");
@@ -84,7 +93,7 @@ class osSARTSolver : public solverBase<ARRAY> {
 
     std::vector<int> isubsets(S);
     for (int i = 0; i < S; ++i) isubsets[i] = i;
-    for (int it = 0; it < iterations_; ++it) {
+    auto iteration = [&](int it) {
       for (int is = 0; is < S; ++is) {
         const int s = isubsets[is];
         encoding_operator_->mult_M(x.get(), tmp_projections[s].get(), s, false);
@@ -97,6 +106,16 @@ class osSARTSolver : public solverBase<ARRAY> {
         // x += beta/(1+gamma i) * tmp_image .* W_x ; clamp   (:99-105)
         sart_update(x.get(), &tmp_image, &ones_images[s], float(beta_ / (1 + gamma_ * it)),
                     non_negativity_);
+      }
+    };
+    std::unique_ptr<StreamGraph> graph[2];   // subsets in order, reversed
+    for (int it = 0; it < iterations_; ++it) {
+      if (graphs && it > 0) {
+        auto &gr = graph[isubsets.front() == 0 ? 0 : 1];
+        if (!gr) gr = std::make_unique<StreamGraph>([&] { iteration(it); });
+        gr->launch();
+      } else {
+        iteration(it);
       }
       std::reverse(isubsets.begin(), isubsets.end());
       if (objective_cb_) {
@@ -119,6 +138,7 @@ class osSARTSolver : public solverBase<ARRAY> {
   float beta_ = 1.0f, gamma_ = 0.0f;
   bool non_negativity_ = false;
   bool dump_ = false;
+  bool graphs_ = false;
   std::function<void(int, double)> objective_cb_;
   std::shared_ptr<subsetOperator<ARRAY>> encoding_operator_;
 };

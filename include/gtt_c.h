@@ -43,6 +43,15 @@ int gtt_set_default_storage(int dtype);
  * reference counterpart. */
 int gtt_set_autotune(int on);
 
+/* Whether the OS-SART reconstructions the calling thread starts from now on
+ * replay their iterations 2.. as captured HIP graphs (one per subset order;
+ * with gamma = 0 only, else they run directly).  Bitwise the same volumes.
+ * Default off.  No reference counterpart. */
+int gtt_set_graphs(int on);
+/* HIP graphs the calling thread has launched so far (for checking that a
+ * reconstruction ran on the graph path). */
+long long gtt_graph_launches(void);
+
 int gtt_fdk_reconstruct_3d(const cbct_geom *g, const float *angles_deg, const float *offsets,
                            int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
                            const float *projections, const float *filter /* nu+1 or NULL */,

@@ -196,6 +196,45 @@ def test_os_sart_matches_oracle(nonneg):
     assert obj[-1] < obj[0]
 
 
+@pytest.mark.parametrize("case", ["plain", "one_subset", "offsets", "weights", "bf16", "gamma"])
+def test_os_sart_graphs_bitwise(case):
+    """OS-SART with iterations 2.. replayed as captured HIP graphs (one per
+    subset order) gives the eager volume and the eager objective after every
+    iteration, bit for bit.  The objective is eager work with a host read
+    between the replays, so every replay is checked; odd and even iteration
+    counts run both subset orders.  'offsets': half-fan offsets (a device
+    copy and the offset weights inside the graph); 'weights': the weighted
+    subset operator; 'bf16': BF16 working copies; 'gamma': a step size that
+    changes per iteration (graphs off, eager path)."""
+    g, angles, offs, vol, p = problem(noise=True, seed=11)
+    if case == "offsets":   # a half-fan scan: the offset weights run inside the graph
+        offs = np.zeros((len(angles), 2), np.float32)
+        offs[:, 0] = 0.2 * g.fov_u
+        p = O.fp(to_oracle(g), angles, offs, vol).astype(np.float32)
+    S, iters, gamma = {"one_subset": (1, 3, 0.0), "gamma": (4, 3, 0.1)}.get(case, (4, 5, 0.0))
+    w = None
+    if case == "weights":
+        w = (0.5 + np.random.default_rng(4).random(p.shape)).astype(np.float32)
+    print("CASE os-sart graphs", case, S, iters, flush=True)
+    if case == "bf16":
+        G.set_default_storage("bf16")
+    try:
+        x0, o0 = G.os_sart(g, angles, offs, p, S, iters, gamma=gamma, nonneg=True, objective=True, weights=w)
+        G.set_graphs(True)
+        n0 = G.graph_launches()
+        x1, o1 = G.os_sart(g, angles, offs, p, S, iters, gamma=gamma, nonneg=True, objective=True, weights=w)
+        x2 = G.os_sart(g, angles, offs, p, S, iters, gamma=gamma, nonneg=True, weights=w)
+        launched = G.graph_launches() - n0
+    finally:
+        G.set_graphs(False)
+        G.set_default_storage("f32")
+    print("objective eager", o0, "graphs", o1)
+    assert np.array_equal(np.asarray(o0), np.asarray(o1))
+    assert np.array_equal(x0, x1)
+    assert np.array_equal(x0, x2)   # replays back to back, no host work between them
+    assert launched == (0 if case == "gamma" else 2 * (iters - 1))
+
+
 @pytest.mark.parametrize("tv", [0.0, 1e-3])
 def test_os_mom_matches_oracle(tv):
     g, angles, offs, vol, p = problem(noise=True, seed=5)
