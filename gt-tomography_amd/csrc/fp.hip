@@ -432,21 +432,16 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
           for (int c = 0; c < NCH; ++c) D[c] = S[c];
         };
         const float *zb = zl - zb0;
-        // one step: the cell's corner lines (loaded or kept), the combined
-        // z-line into LDS, every ray's z-lerp from it.  BF16 copies reload all
-        // four lines on a cell change, straight into their registers (fewer
-        // instructions than keeping the shared lines, whose register moves
-        // wait on the new loads); FP32 copies, bound by their line bytes,
-        // load only the lines the new cell does not share.
-        auto step = [&](const float fm, const float ax, const float ay, const int ix,
-                        const int iy) __attribute__((always_inline)) {
-            if (sizeof(T) == 2 && (ix != cx || iy != cy)) {
-              const unsigned B = ((unsigned)iy * P.px + (unsigned)ix) * PZ * esz + zoff;
-              load_line(L00, B); load_line(L10, B + so_x);
-              load_line(L01, B + so_y); load_line(L11, B + so_xy);
-              cx = ix;
-              cy = iy;
-            } else if (ix != cx || iy != cy) {
+        // all four lines of cell (ix, iy)
+        auto load_cell = [&](int ix, int iy) __attribute__((always_inline)) {
+          const unsigned B = ((unsigned)iy * P.px + (unsigned)ix) * PZ * esz + zoff;
+          load_line(L00, B); load_line(L10, B + so_x);
+          load_line(L01, B + so_y); load_line(L11, B + so_xy);
+        };
+        // the four lines of cell (ix, iy), keeping those shared with the
+        // current cell (FP32 copies, bound by their line bytes)
+        auto update_cell = [&](int ix, int iy) __attribute__((always_inline)) {
+            if (ix != cx || iy != cy) {
               const int dx = ix - cx, dy = iy - cy;
               const unsigned B = ((unsigned)iy * P.px + (unsigned)ix) * PZ * esz + zoff;
               if (dy == 0 && dx == 1) {
@@ -480,6 +475,9 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
               cx = ix;
               cy = iy;
             }
+        };
+        // the combined z-line of the cell's lines into LDS
+        auto combine = [&](const float ax, const float ay) __attribute__((always_inline)) {
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
               const float2 v00 = PairWord<T>::unpack(L00[c]), v10 = PairWord<T>::unpack(L10[c]);
@@ -498,6 +496,9 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
               *reinterpret_cast<float2 *>(&zl[2 * lane + 128 * c]) = o;
             }
             wave_lds_sync();
+        };
+        // every ray's z-lerp from the combined z-line
+        auto rays = [&](const float fm) __attribute__((always_inline)) {
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
               const float tz = fmaf(fm, qz[r], pz[r]);  // in [0, nz+1]: checked above
@@ -508,24 +509,55 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
             }
             wave_lds_sync();
         };
-        // The x/y part of a step is the same for the whole wave.  BF16
-        // copies: it is evaluated for 64 steps at once, step mb + lane in
-        // each lane (the same expressions as per step, so the same values),
-        // and each step reads its cell and weights back into scalar
-        // registers.  FP32 copies: per step (measured faster there).
-        for (int mb = 0; sizeof(T) == 2 && mb < P.n; mb += 64) {
-          const float fml = (float)(mb + lane);
-          const float sxl = __builtin_amdgcn_fmed3f(fmaf(fml, qx, px0), 0.0f, P.cx);
-          const float syl = __builtin_amdgcn_fmed3f(fmaf(fml, qy, py0), 0.0f, P.cy);
-          const int axl = __float_as_int(__builtin_amdgcn_fractf(sxl));
-          const int ayl = __float_as_int(__builtin_amdgcn_fractf(syl));
-          const int cl = (int)((unsigned)sxl | ((unsigned)syl << 16));
-          const int cnt = __builtin_amdgcn_readfirstlane(min(64, P.n - mb));
-          for (int l = 0; l < cnt; ++l) {
-            const unsigned c = (unsigned)__builtin_amdgcn_readlane(cl, l);
-            const float ax = __int_as_float(__builtin_amdgcn_readlane(axl, l));
-            const float ay = __int_as_float(__builtin_amdgcn_readlane(ayl, l));
-            step((float)(mb + l), ax, ay, (int)(c & 0xffffu), (int)(c >> 16));
+        auto step = [&](const float fm, const float ax, const float ay, const int ix,
+                        const int iy) __attribute__((always_inline)) {
+          update_cell(ix, iy);
+          combine(ax, ay);
+          rays(fm);
+        };
+        // BF16 copies: the x/y part of a step, the same for the whole wave,
+        // is evaluated for 64 steps at once, step mb + lane in each lane (the
+        // same expressions as per step, so the same values), and each step
+        // reads its cell and weights back into scalar registers.  The lines
+        // of the next step's cell, when it changes, are all four reloaded
+        // straight into their registers once this step's combine has used
+        // them, so they arrive while this step's rays run (fewer
+        // instructions than keeping the shared lines, whose register moves
+        // wait on the new loads).  FP32 copies: per-step setup and shared
+        // lines kept (measured faster there).
+        if constexpr (sizeof(T) == 2) {
+          {
+            const float sx = __builtin_amdgcn_fmed3f(fmaf(0.0f, qx, px0), 0.0f, P.cx);
+            const float sy = __builtin_amdgcn_fmed3f(fmaf(0.0f, qy, py0), 0.0f, P.cy);
+            cx = __builtin_amdgcn_readfirstlane((int)sx);
+            cy = __builtin_amdgcn_readfirstlane((int)sy);
+            load_cell(cx, cy);
+          }
+          for (int mb = 0; mb < P.n; mb += 64) {
+            const float fml = (float)(mb + lane);
+            const float sxl = __builtin_amdgcn_fmed3f(fmaf(fml, qx, px0), 0.0f, P.cx);
+            const float syl = __builtin_amdgcn_fmed3f(fmaf(fml, qy, py0), 0.0f, P.cy);
+            const int axl = __float_as_int(__builtin_amdgcn_fractf(sxl));
+            const int ayl = __float_as_int(__builtin_amdgcn_fractf(syl));
+            // cell of step mb + lane + 1 (the next step's)
+            const float fmn = (float)(mb + lane + 1);
+            const float sxn = __builtin_amdgcn_fmed3f(fmaf(fmn, qx, px0), 0.0f, P.cx);
+            const float syn = __builtin_amdgcn_fmed3f(fmaf(fmn, qy, py0), 0.0f, P.cy);
+            const int cn = (int)((unsigned)sxn | ((unsigned)syn << 16));
+            const int cnt = __builtin_amdgcn_readfirstlane(min(64, P.n - mb));
+            for (int l = 0; l < cnt; ++l) {
+              const float ax = __int_as_float(__builtin_amdgcn_readlane(axl, l));
+              const float ay = __int_as_float(__builtin_amdgcn_readlane(ayl, l));
+              combine(ax, ay);
+              const unsigned c = (unsigned)__builtin_amdgcn_readlane(cn, l);
+              const int nx = (int)(c & 0xffffu), ny = (int)(c >> 16);
+              if (mb + l + 1 < P.n && (nx != cx || ny != cy)) {
+                load_cell(nx, ny);
+                cx = nx;
+                cy = ny;
+              }
+              rays((float)(mb + l));
+            }
           }
         }
         for (int m = 0; sizeof(T) != 2 && m < P.n; ++m) {
