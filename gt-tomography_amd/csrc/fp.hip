@@ -315,9 +315,10 @@ __device__ __forceinline__ void col_block(const FPParams &P, unsigned W, unsigne
 //     of four for a move along x or y, one for a diagonal move) and loads
 //     only the others; the column's (x, y) is wave-uniform, so the case
 //     split is a scalar branch, and the lines are kept as (00, 10, 01, 11)
-//     of the cell and shifted by register moves.  BF16 copies: a cell change
-//     reloads all four.  Every line load fetches all NCH chunks (no
-//     per-chunk branches); chunks past the window are never read.
+//     of the cell and shifted by register moves.  BF16 copies: all four are
+//     reloaded for the next step's cell right after this step's combine.
+//     Every line load fetches all NCH chunks (no per-chunk branches); chunks
+//     past the window are never read.
 // All produce the same bits as the per-ray engine.
 template <typename T, int RPL, bool SSTEP, int ZP, int NCH>
 __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__restrict__ views,
@@ -509,6 +510,7 @@ __device__ __forceinline__ void fp_col_body(const FPParams &P, const ViewD *__re
             }
             wave_lds_sync();
         };
+        // one step with FP32 copies
         auto step = [&](const float fm, const float ax, const float ay, const int ix,
                         const int iy) __attribute__((always_inline)) {
           update_cell(ix, iy);
@@ -737,8 +739,8 @@ __global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restri
   fp_col_body<T, RPL, SSTEP, ZP, 0>(P, views, vt, proj, accumulate, ngroups, nblocks, vt_bytes);
 }
 
-// The cached variant, held to five waves per SIMD (it is latency-bound: five
-// waves measured faster than four with more registers).
+// The cached variant, held to five waves per SIMD (five measured faster than
+// four with more registers, and than six with BF16 copies).
 template <typename T, int RPL, int NCH>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 k_fp_colc(FPParams P, const ViewD *__restrict__ views, const T *__restrict__ vt,
