@@ -362,45 +362,71 @@ __device__ __forceinline__ unsigned short to_storage<unsigned short, float>(floa
   return f32_to_bf16(v);
 }
 
+// The four taps of quad texel (a, b) as one store: 16 bytes (FP32), 8 (BF16).
+template <typename T>
+__device__ __forceinline__ void store_quad(T *o, const T (&q)[4]);
+template <>
+__device__ __forceinline__ void store_quad<float>(float *o, const float (&q)[4]) {
+  *reinterpret_cast<float4 *>(o) = make_float4(q[0], q[1], q[2], q[3]);
+}
+template <>
+__device__ __forceinline__ void store_quad<unsigned short>(unsigned short *o,
+                                                           const unsigned short (&q)[4]) {
+  *reinterpret_cast<uint2 *>(o) = make_uint2((unsigned)q[0] | ((unsigned)q[1] << 16),
+                                             (unsigned)q[2] | ((unsigned)q[3] << 16));
+}
+
+// Padded copies: each wave covers 64 texel columns a (blockIdx.x) of one row
+// b, strided over rows by the block's four waves and gridDim.y, and over
+// views by gridDim.z, so no thread divides and each does many texels.
 // TI: the caller's projection type; T: the copy's (FP32 -> BF16 converts).
 template <typename TI, typename T>
 __global__ void __launch_bounds__(256) k_pad_quad(const TI *__restrict__ in, T *__restrict__ out,
                                                   int nu, int nv, unsigned pnu, unsigned pnv,
-                                                  size_t total) {
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
-       t += (size_t)gridDim.x * blockDim.x) {
-    const int a = (int)(t % pnu);
-    const int b = (int)((t / pnu) % pnv);
-    const size_t k = t / ((size_t)pnu * pnv);
+                                                  int nviews) {
+  const int a = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+  if (a >= (int)pnu) return;
+  const int b0 = (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), bs = (int)(gridDim.y * 4);
+  for (int k = (int)blockIdx.z; k < nviews; k += (int)gridDim.z) {
     const TI *pk = in + (size_t)nu * nv * k;
-    T q[4];
+    T *ok = out + 4 * (size_t)pnu * pnv * k;
+    for (int b = b0; b < (int)pnv; b += bs) {
+      T q[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int u = a - 1 + (e & 1), v = b - 1 + (e >> 1);
-      q[e] = (u >= 0 && u < nu && v >= 0 && v < nv) ? to_storage<T>(pk[(size_t)u + (size_t)nu * v])
-                                                     : T(0);
+      for (int e = 0; e < 4; ++e) {
+        const int u = a - 1 + (e & 1), v = b - 1 + (e >> 1);
+        q[e] = (u >= 0 && u < nu && v >= 0 && v < nv) ? to_storage<T>(pk[(size_t)u + (size_t)nu * v])
+                                                       : T(0);
+      }
+      store_quad<T>(ok + 4 * ((size_t)pnu * b + a), q);
     }
-    T *o = out + 4 * t;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) o[e] = q[e];
   }
 }
 
+// Launch grid of the padded copies (see k_pad_quad).
+inline dim3 pad_grid(unsigned pnu, int nviews) {
+  return dim3((pnu + 63) / 64, 16, (unsigned)std::min(nviews, 1024));
+}
+
 // Plain zero-bordered copy with the storage conversion (FP32 -> BF16).
+// Grid as k_pad_quad.
 template <typename TI, typename T>
 __global__ void __launch_bounds__(256) k_pad_plain(const TI *__restrict__ in, T *__restrict__ out,
                                                    int nu, int nv, unsigned pnu, unsigned pnv,
-                                                   size_t total) {
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
-       t += (size_t)gridDim.x * blockDim.x) {
-    const unsigned a = (unsigned)(t % pnu);
-    const unsigned b = (unsigned)((t / pnu) % pnv);
-    const size_t k = t / ((size_t)pnu * pnv);
-    const int u = (int)a - 1, v = (int)b - 1;
-    T val = T(0);
-    if (u >= 0 && u < nu && v >= 0 && v < nv)
-      val = to_storage<T>(in[(size_t)u + (size_t)nu * ((size_t)v + (size_t)nv * k)]);
-    out[t] = val;
+                                                   int nviews) {
+  const int a = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+  if (a >= (int)pnu) return;
+  const int b0 = (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), bs = (int)(gridDim.y * 4);
+  const int u = a - 1;
+  for (int k = (int)blockIdx.z; k < nviews; k += (int)gridDim.z) {
+    const TI *pk = in + (size_t)nu * nv * k;
+    T *ok = out + (size_t)pnu * pnv * k;
+    for (int b = b0; b < (int)pnv; b += bs) {
+      const int v = b - 1;
+      T val = T(0);
+      if (u >= 0 && u < nu && v >= 0 && v < nv) val = to_storage<T>(pk[(size_t)u + (size_t)nu * v]);
+      ok[(size_t)pnu * b + a] = val;
+    }
   }
 }
 
@@ -500,8 +526,8 @@ int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int v
     int rc2 = cbct_reserve(ctx, ctx->proj_pad, totalp * sizeof(T));
     if (rc2) return rc2;
     T *pp2 = static_cast<T *>(ctx->proj_pad.ptr);
-    hipLaunchKernelGGL((k_pad_plain<TI, T>), stride_grid(totalp), dim3(256), 0, s, proj, pp2, g->nu, g->nv,
-                       P.pnu, P.pnv, totalp);
+    hipLaunchKernelGGL((k_pad_plain<TI, T>), pad_grid(P.pnu, nviews), dim3(256), 0, s, proj, pp2,
+                       g->nu, g->nv, P.pnu, P.pnv, nviews);
     CBCT_LAUNCH_CHECK(ctx, "k_pad_plain");
     cbct_prof_mark(ctx, PROF_BP, 0, s);
     const unsigned nblocks = grid.x * grid.y * grid.z;
@@ -523,8 +549,8 @@ int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int v
   if (rc) return rc;
   T *pp = static_cast<T *>(ctx->proj_pad.ptr);
   if (quad) {
-    hipLaunchKernelGGL((k_pad_quad<TI, T>), stride_grid(total), dim3(256), 0, s, proj, pp, g->nu,
-                       g->nv, pnu, pnv, total);
+    hipLaunchKernelGGL((k_pad_quad<TI, T>), pad_grid(pnu, nviews), dim3(256), 0, s, proj, pp, g->nu,
+                       g->nv, pnu, pnv, nviews);
   } else {
     hipLaunchKernelGGL((k_pad_proj<T, T>), stride_grid(total), dim3(256), 0, s,
                        reinterpret_cast<const T *>(proj), pp, g->nu, g->nv, pnu, pnv, total);
