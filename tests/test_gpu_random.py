@@ -6,7 +6,8 @@ For every case:
     launch groups of adjacent views) gives the same bits;
   * FP, and BP plain and FDK-weighted, match the FP64 oracle within
     rel-L2 1e-5;
-  * the backprojector engines give the same bits.
+  * the backprojector engines give the same bits;
+  * with BF16 working copies, the engines again agree bit for bit.
 """
 import numpy as np
 import pytest
@@ -64,26 +65,43 @@ def test_random_geometry_parity(ctx, seed):
     print("CASE random", seed, g, len(angles), "offsets" if offsets is not None else "", flush=True)
     vol = phantom_volume(g, seed=seed) + 0.01
     ctx.set_views(angles, offsets)
-    v = dev(vol)
-    outs = []
-    try:
-        for e, vg in ((0, 6), (1, 1), (1, 6), (2, 6), (3, 5), (4, 3)):
-            ctx.fp_engine(e)
-            ctx.fp_view_group(vg)
-            out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
-            ctx.fp(g, v, out)
-            outs.append(out)
-    finally:
-        ctx.fp_engine(1)
-        ctx.fp_view_group(6)
-    for i in range(1, len(outs)):
-        assert torch.equal(outs[0], outs[i]), i
-    err = rel_l2(host(outs[2]), O.fp(to_oracle(g), angles, offsets, vol))
+
+    def fp_all(v):
+        outs = []
+        try:
+            for e, vg in ((0, 6), (1, 1), (1, 6), (2, 6), (3, 5), (4, 3)):
+                ctx.fp_engine(e)
+                ctx.fp_view_group(vg)
+                out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
+                ctx.fp(g, v, out)
+                outs.append(out)
+        finally:
+            ctx.fp_engine(1)
+            ctx.fp_view_group(6)
+        for i in range(1, len(outs)):
+            assert torch.equal(outs[0], outs[i]), i
+        return outs[2]
+
+    fp_all(dev(vol).to(torch.bfloat16))
+    out = fp_all(dev(vol))
+    err = rel_l2(host(out), O.fp(to_oracle(g), angles, offsets, vol))
     print("fp rel-L2", err)
     assert err <= TOL
 
     proj = np.random.default_rng(seed).random(g.proj_shape(len(angles)))
     p = dev(proj)
+    try:   # BF16 working copies: the engines agree
+        ctx.storage("bf16")
+        b = []
+        for e in (0, 1, 2):
+            ctx.bp_engine(e)
+            out = torch.empty(g.vol_shape, device="cuda")
+            ctx.bp(g, p, out, fbp=True)
+            b.append(out)
+        assert torch.equal(b[0], b[1]) and torch.equal(b[0], b[2])
+    finally:
+        ctx.storage("f32")
+        ctx.bp_engine(2)
     for fbp in (False, True):
         bouts = []
         try:
