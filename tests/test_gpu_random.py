@@ -116,3 +116,40 @@ def test_random_geometry_parity(ctx, seed):
         err = rel_l2(host(bouts[2]), O.bp(to_oracle(g), angles, offsets, proj, fbp=fbp))
         print("bp rel-L2", fbp, err)
         assert err <= TOL
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_os_sart_parity(seed):
+    """OS-SART on seeded random geometries (random subset counts, iterations
+    and constraint; iterations replayed as HIP graphs) against the oracle
+    within rel-L2 1e-4."""
+    from gtt import operators as G
+    r = np.random.default_rng(2000 + seed)
+    # the solvers' geometry family (square slices, a detector 1.56 times the
+    # volume's width, a full arc): random sizes, views, subsets and
+    # iterations.  Short scans and narrow detectors leave voxels and rays
+    # that OS-SART's normalisations amplify, testing conditioning, not parity
+    from helpers import geom
+    n = int(r.integers(10, 25))
+    g = geom(n, n, int(r.integers(6, 25)), int(r.integers(20, 48)), int(r.integers(14, 40)))
+    nviews = int(r.integers(8, 28))
+    angles = angles_full(nviews, 360.0, start=float(r.uniform(0.0, 360.0)))
+    offsets = None
+    S = int(r.integers(1, 5))
+    iters = int(r.integers(2, 4))
+    nonneg = bool(r.random() < 0.5)
+    print("CASE random os-sart", seed, g, len(angles), S, iters, nonneg, flush=True)
+    vol = phantom_volume(g, seed=seed)
+    p = O.fp(to_oracle(g), angles, offsets, vol).astype(np.float32)
+    G.set_graphs(True)
+    try:
+        x = G.os_sart(g, angles, offsets, p, S, iters, beta=0.9, nonneg=nonneg)
+    finally:
+        G.set_graphs(False)
+    op = O.CBSubsetOperator(S)
+    o = np.zeros((len(angles), 2), np.float32) if offsets is None else offsets
+    pb = op.setup(to_oracle(g), angles, o, p, O.Binning.default_3d(len(angles)))
+    ref = O.os_sart(op, pb, iters, beta=0.9, nonneg=nonneg)
+    err = rel_l2(x, ref)
+    print("os-sart rel-L2", err)
+    assert err <= 1e-4
