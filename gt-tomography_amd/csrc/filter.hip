@@ -166,7 +166,8 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int MB = 128, NB = 128, KB = 32;
-constexpr int LDK = KB + 8;  // LDS row pitch in halves (80 bytes: 16-byte aligned, staggered banks)
+constexpr int SEGV = KB / 16;   // 16-byte vectors per thread and tile row half
+constexpr int LDK = KB + 8;  // LDS row pitch in halves (16-byte aligned rows, staggered banks)
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -230,7 +231,9 @@ __global__ void __launch_bounds__(256) k_split_rows(const float *__restrict__ in
 // 256 threads: four waves in a 2x2 arrangement, each owning a 64x64 output
 // block as 2x2 tiles of the 32x32x16 fp16 MFMA.  K advances in steps of 32
 // through LDS; the next step's global loads are issued before the MFMAs.
-__global__ void __launch_bounds__(256) k_toeplitz_mfma(const _Float16 *__restrict__ xh,
+// At most two waves per SIMD: with the register budget of more, the compiler
+// put part of the kernel's state in scratch memory (half the speed).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) k_toeplitz_mfma(const _Float16 *__restrict__ xh,
                                                        const _Float16 *__restrict__ xl,
                                                        const float *__restrict__ rinv,
                                                        const _Float16 *__restrict__ th,
@@ -246,15 +249,18 @@ __global__ void __launch_bounds__(256) k_toeplitz_mfma(const _Float16 *__restric
   const int n0 = (int)tn * NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  // loader: thread -> (row, 16-half segment) of the 128 x 32 tiles
-  const int lr = tid >> 1, lc = (tid & 1) * 16;
+  // loader: thread -> (row, half of the KB columns) of the 128 x KB tiles
+  const int lr = tid >> 1, lc = (tid & 1) * (KB / 2);
   const uint4 *gah = reinterpret_cast<const uint4 *>(xh + (m0 + lr) * kp + lc);
   const uint4 *gal = reinterpret_cast<const uint4 *>(xl + (m0 + lr) * kp + lc);
   const uint4 *gbh = reinterpret_cast<const uint4 *>(th + (size_t)(n0 + lr) * kp + lc);
   const uint4 *gbl = reinterpret_cast<const uint4 *>(tl + (size_t)(n0 + lr) * kp + lc);
-  uint4 ra[4], rb[4];
-  ra[0] = gah[0]; ra[1] = gah[1]; ra[2] = gal[0]; ra[3] = gal[1];
-  rb[0] = gbh[0]; rb[1] = gbh[1]; rb[2] = gbl[0]; rb[3] = gbl[1];
+  uint4 ra[2 * SEGV], rb[2 * SEGV];
+#pragma unroll
+  for (int j = 0; j < SEGV; ++j) {
+    ra[j] = gah[j]; ra[SEGV + j] = gal[j];
+    rb[j] = gbh[j]; rb[SEGV + j] = gbl[j];
+  }
   floatx16 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -270,13 +276,19 @@ __global__ void __launch_bounds__(256) k_toeplitz_mfma(const _Float16 *__restric
     uint4 *dal = reinterpret_cast<uint4 *>(&sa[1][lr * LDK + lc]);
     uint4 *dbh = reinterpret_cast<uint4 *>(&sb[0][lr * LDK + lc]);
     uint4 *dbl = reinterpret_cast<uint4 *>(&sb[1][lr * LDK + lc]);
-    dah[0] = ra[0]; dah[1] = ra[1]; dal[0] = ra[2]; dal[1] = ra[3];
-    dbh[0] = rb[0]; dbh[1] = rb[1]; dbl[0] = rb[2]; dbl[1] = rb[3];
+#pragma unroll
+    for (int j = 0; j < SEGV; ++j) {
+      dah[j] = ra[j]; dal[j] = ra[SEGV + j];
+      dbh[j] = rb[j]; dbl[j] = rb[SEGV + j];
+    }
     __syncthreads();
     if (k0 + KB < kp) {
       const int o = (k0 + KB) / 8;  // uint4 = 8 halves
-      ra[0] = gah[o]; ra[1] = gah[o + 1]; ra[2] = gal[o]; ra[3] = gal[o + 1];
-      rb[0] = gbh[o]; rb[1] = gbh[o + 1]; rb[2] = gbl[o]; rb[3] = gbl[o + 1];
+#pragma unroll
+      for (int j = 0; j < SEGV; ++j) {
+        ra[j] = gah[o + j]; ra[SEGV + j] = gal[o + j];
+        rb[j] = gbh[o + j]; rb[SEGV + j] = gbl[o + j];
+      }
     }
 #pragma unroll
     for (int kk = 0; kk < KB; kk += 16) {
