@@ -497,27 +497,39 @@ BPParams make_params(const cbct_geom *g, int nviews) {
 
 // z-voxels per thread: the per-(x, y, view) work (rotation, magnification,
 // u coordinate and weight, FDK weight) is shared by KZ voxel-views (measured:
-// 16 beats 8 by 4-5% and 32 by a third at 512^3)
+// 16 beats 8 by 4-5% and 32 by a third at 512^3).  8 when 16 would make
+// fewer tiles than the chip holds blocks at once (1024 at four per CU): a
+// 128^3 volume makes 512 tiles of 16 z.
 constexpr int KZ = 16;
+inline int bp_kz(const cbct_geom *g) {
+  const size_t tiles16 = (size_t)ceil_div(g->nx, BP_TX) * ceil_div(g->ny, BP_TY) * ceil_div(g->nz, 16);
+  return tiles16 < 1024 ? 8 : KZ;
+}
 
-template <typename T, bool QUAD>
-void launch_bp(const BPParams &P, dim3 grid, hipStream_t s, const ViewF *views, const T *pp,
-               float *vol, int fbp, int acc) {
+template <typename T, bool QUAD, int Z>
+void launch_bp_kz(const BPParams &P, dim3 grid, hipStream_t s, const ViewF *views, const T *pp,
+                  float *vol, int fbp, int acc) {
   dim3 block(BP_TX, BP_TY);
   const unsigned nblocks = grid.x * grid.y * grid.z;
   if (fbp)
-    hipLaunchKernelGGL((k_bp<T, true, KZ, QUAD>), dim3(nblocks), block, 0, s, P, views, pp, vol, acc,
+    hipLaunchKernelGGL((k_bp<T, true, Z, QUAD>), dim3(nblocks), block, 0, s, P, views, pp, vol, acc,
                        grid.x, grid.y);
   else
-    hipLaunchKernelGGL((k_bp<T, false, KZ, QUAD>), dim3(nblocks), block, 0, s, P, views, pp, vol,
+    hipLaunchKernelGGL((k_bp<T, false, Z, QUAD>), dim3(nblocks), block, 0, s, P, views, pp, vol,
                        acc, grid.x, grid.y);
+}
+template <typename T, bool QUAD>
+void launch_bp(const BPParams &P, dim3 grid, int kz, hipStream_t s, const ViewF *views, const T *pp,
+               float *vol, int fbp, int acc) {
+  if (kz == 8) launch_bp_kz<T, QUAD, 8>(P, grid, s, views, pp, vol, fbp, acc);
+  else launch_bp_kz<T, QUAD, KZ>(P, grid, s, views, pp, vol, fbp, acc);
 }
 
 // Padded copy in the selected layout (and storage type), then the kernel.
 // TI: the caller's projection type; T: the copy's.
 template <typename TI, typename T>
-int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int view0, int nviews,
-           const TI *proj, float *vol, int fbp, int accumulate, hipStream_t s) {
+int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int kz, int view0,
+           int nviews, const TI *proj, float *vol, int fbp, int accumulate, hipStream_t s) {
   // engine 2: the LDS-staged kernel on a plain copy for FP32 working copies
   // (BF16 ones use quads, measured faster for them); engine 0's plain layout
   // converts nothing: BF16 working copies use quads
@@ -532,12 +544,19 @@ int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int v
     cbct_prof_mark(ctx, PROF_BP, 0, s);
     const unsigned nblocks = grid.x * grid.y * grid.z;
     const float *pf = reinterpret_cast<const float *>(pp2);
-    if (fbp)
-      hipLaunchKernelGGL((k_bps<true, KZ>), dim3(nblocks), dim3(BP_TX, BP_TY), 0, s, P,
-                         ctx->d_views_f + view0, pf, vol, accumulate, grid.x, grid.y);
-    else
-      hipLaunchKernelGGL((k_bps<false, KZ>), dim3(nblocks), dim3(BP_TX, BP_TY), 0, s, P,
-                         ctx->d_views_f + view0, pf, vol, accumulate, grid.x, grid.y);
+    const ViewF *vf = ctx->d_views_f + view0;
+    const dim3 blk(BP_TX, BP_TY);
+    if (kz == 8) {
+      if (fbp)
+        hipLaunchKernelGGL((k_bps<true, 8>), dim3(nblocks), blk, 0, s, P, vf, pf, vol, accumulate, grid.x, grid.y);
+      else
+        hipLaunchKernelGGL((k_bps<false, 8>), dim3(nblocks), blk, 0, s, P, vf, pf, vol, accumulate, grid.x, grid.y);
+    } else {
+      if (fbp)
+        hipLaunchKernelGGL((k_bps<true, KZ>), dim3(nblocks), blk, 0, s, P, vf, pf, vol, accumulate, grid.x, grid.y);
+      else
+        hipLaunchKernelGGL((k_bps<false, KZ>), dim3(nblocks), blk, 0, s, P, vf, pf, vol, accumulate, grid.x, grid.y);
+    }
     cbct_prof_mark(ctx, PROF_BP, 1, s);
     CBCT_LAUNCH_CHECK(ctx, "k_bps");
     return CBCT_OK;
@@ -558,9 +577,9 @@ int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int v
   CBCT_LAUNCH_CHECK(ctx, "k_pad_proj");
   cbct_prof_mark(ctx, PROF_BP, 0, s);
   if (quad)
-    launch_bp<T, true>(P, grid, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
+    launch_bp<T, true>(P, grid, kz, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
   else
-    launch_bp<T, false>(P, grid, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
+    launch_bp<T, false>(P, grid, kz, s, ctx->d_views_f + view0, pp, vol, fbp, accumulate);
   cbct_prof_mark(ctx, PROF_BP, 1, s);
   CBCT_LAUNCH_CHECK(ctx, "k_bp");
   return CBCT_OK;
@@ -587,7 +606,8 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
   BPParams P = make_params(g, nviews);
-  dim3 grid(ceil_div(g->nx, BP_TX), ceil_div(g->ny, BP_TY), ceil_div(g->nz, KZ));
+  const int kz = bp_kz(g);
+  dim3 grid(ceil_div(g->nx, BP_TX), ceil_div(g->ny, BP_TY), ceil_div(g->nz, kz));
   if (nviews == 0) {
     // No views: the reference clears the image unless accumulating
     // (xray/conebeam_projection.cu:1343-1350).  A fill kernel, not a memset,
@@ -600,13 +620,13 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "Error: %s: detector too large (%d x %d)", who,
                      g->nu, g->nv);
   if (dtype == CBCT_BF16)
-    return run_bp<unsigned short, unsigned short>(ctx, g, P, grid, view0, nviews,
+    return run_bp<unsigned short, unsigned short>(ctx, g, P, grid, kz, view0, nviews,
                                                   static_cast<const unsigned short *>(proj), vol,
                                                   fbp, accumulate, s);
   if (ctx->work_dtype == CBCT_BF16)
-    return run_bp<float, unsigned short>(ctx, g, P, grid, view0, nviews,
+    return run_bp<float, unsigned short>(ctx, g, P, grid, kz, view0, nviews,
                                          static_cast<const float *>(proj), vol, fbp, accumulate, s);
-  return run_bp<float, float>(ctx, g, P, grid, view0, nviews, static_cast<const float *>(proj),
+  return run_bp<float, float>(ctx, g, P, grid, kz, view0, nviews, static_cast<const float *>(proj),
                               vol, fbp, accumulate, s);
 }
 
