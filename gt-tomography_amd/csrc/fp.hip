@@ -878,16 +878,25 @@ template <typename T, int RPL, bool SSTEP, int ZP, int NCH>
 void launch_col(const FPParams &P, hipStream_t s, const ViewD *views, const T *vt, float *proj,
                 int acc, int nviews, unsigned vt_bytes) {
   const unsigned ngroups = (unsigned)((P.nv + 64 * RPL - 1) / (64 * RPL));
-  const unsigned nblocks = (unsigned)P.nu * ngroups * (unsigned)nviews;
   // one z-line; the cached variants write all NCH chunks of their window
   const size_t zline = ((P.pz + 127) / 128) * 128;
   const size_t lds = (zline > (size_t)128 * NCH ? zline : (size_t)128 * NCH) * sizeof(float);
-  if constexpr (NCH > 0)
-    hipLaunchKernelGGL((k_fp_colc<T, RPL, NCH>), dim3(nblocks), dim3(64), lds, s, P, views, vt, proj,
-                       acc, ngroups, nblocks, vt_bytes);
-  else
-    hipLaunchKernelGGL((k_fp_col<T, RPL, SSTEP, ZP>), dim3(nblocks), dim3(64), lds, s, P, views, vt,
-                       proj, acc, ngroups, nblocks, vt_bytes);
+  // a launch covers at most 2^32 - 1 work-items: larger calls go out as
+  // several launches over consecutive views (views are independent)
+  const size_t per_view = (size_t)P.nu * ngroups;
+  const int max_views = (int)std::max<size_t>(1, std::min<size_t>(nviews, ((1ull << 32) - 1) / 64 / per_view));
+  for (int v0 = 0; v0 < nviews; v0 += max_views) {
+    const int nvc = std::min(max_views, nviews - v0);
+    const unsigned nblocks = (unsigned)(per_view * (size_t)nvc);
+    const ViewD *vw = views + v0;
+    float *pj = proj + (size_t)v0 * P.nu * P.nv;
+    if constexpr (NCH > 0)
+      hipLaunchKernelGGL((k_fp_colc<T, RPL, NCH>), dim3(nblocks), dim3(64), lds, s, P, vw, vt, pj, acc,
+                         ngroups, nblocks, vt_bytes);
+    else
+      hipLaunchKernelGGL((k_fp_col<T, RPL, SSTEP, ZP>), dim3(nblocks), dim3(64), lds, s, P, vw, vt, pj,
+                         acc, ngroups, nblocks, vt_bytes);
+  }
 }
 
 // variant 1: vector step setup, one z per load; variant 2: scalar step

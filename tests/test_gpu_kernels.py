@@ -154,6 +154,30 @@ def test_fp_view_groups_bitwise_identical(ctx, dtype):
             ctx.fp_view_group(bad)
 
 
+def test_fp_split_launches(ctx):
+    """A call whose column-engine grid exceeds one launch's 2^32 - 1
+    work-items (an 8192 x 1 detector over 8200 views: 6.7e7 one-wave blocks)
+    goes out as launches over consecutive views, with the per-ray engine's
+    bits."""
+    g = geom(8, 8, 8, 8192, 1)
+    angles = angles_full(8200)
+    print_case("split launches", g, len(angles))
+    vol = phantom_volume(g) + 0.01
+    ctx.set_views(angles, None)
+    outs = []
+    try:
+        for e in (0, 1):
+            ctx.fp_engine(e)
+            out = torch.zeros(g.proj_shape(len(angles)), device="cuda")
+            ctx.fp(g, dev(vol), out)
+            outs.append(out)
+    finally:
+        ctx.fp_engine(1)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    assert float(outs[1].abs().sum()) > 0.0
+
+
 @pytest.mark.parametrize("fbp", [False, True])
 @pytest.mark.parametrize("name", list(CASES))
 def test_bp_parity(ctx, name, fbp):
