@@ -171,6 +171,9 @@ StreamGraph::StreamGraph(const std::function<void()> &work) {
 }
 
 StreamGraph::~StreamGraph() {
+  // the last replay may still run: finish it before the graph and its
+  // blocks are let go
+  if (stream_) (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
   if (exec_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec_));
   if (graph_) (void)hipGraphDestroy(static_cast<hipGraph_t>(graph_));
   // back into the cache: reused only on the same stream, after the replays
