@@ -8,26 +8,55 @@ and the sinogram in BF16, arithmetic in FP32 (--storage f32 for FP32
 copies).  One step = one forward projection of the volume onto all 720
 views + one backprojection of all 720 views, inputs resident in HBM.
 GUPS = 2 * Nvox * Nviews / t_step / 1e9.  The plain FP32 path is timed in
-the same run, with the error of each BF16 stage against it.
+the same run.
 
-Prints one JSON line (rank 0) with the roofline of the dominant kernel and
-a CPU baseline (FP32 OpenMP restatement timed on this host on a bounded
-sample of views).
+Besides the headline step the line carries:
+  roofline      the dominant kernel against the resource that bounds it
+                (issued VALU lane-instructions for the forward projector,
+                measured HBM bytes for the backprojector, MFMA flops for the
+                ramp filter), from the live kernel time (HIP events on the
+                kernel's stream) and the counters committed under profiles/;
+  cpu_baseline  the FP32 OpenMP restatement and the FP64 oracle timed on
+                this host (bounded samples), and each stage's error against
+                the FP64 oracle on a deterministic subsample;
+  workloads     the 128^3 FDK and 256^3 OS-SART / osMOMSolverD workloads;
+  latency       the served-vs-plain reconstruction batch (64 uneven scans)
+                and Poisson request stream.
 """
 import argparse
+import hashlib
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gt-tomography_amd"))
 
+# MI355X published peaks
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+VALU_PEAK_TOPS = FP32_PEAK_TFLOPS / 2.0   # lane-instructions/s: one FMA (2 flops) per lane
+FP16_MFMA_PEAK_TFLOPS = 2500.0
+
+FP_SOURCES = ("gt-tomography_amd/csrc/fp.hip", "gt-tomography_amd/csrc/cbct_internal.h")
+BP_SOURCES = ("gt-tomography_amd/csrc/bp.hip", "gt-tomography_amd/csrc/cbct_internal.h")
+FILTER_SOURCES = ("gt-tomography_amd/csrc/filter.hip", "gt-tomography_amd/csrc/cbct_internal.h")
 
 
-def algorithmic_bytes(kind, nvox, nviews, nrays, nsamp, s_vol, s_proj):
-    """Footprint-convention bytes of one launch.
+def sources_hash(paths):
+    h = hashlib.sha256()
+    for p in paths:
+        with open(os.path.join(ROOT, p), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def footprint_bytes(kind, nvox, nviews, nrays, nsamp, s_vol, s_proj):
+    """Footprint-convention bytes of one launch (an effective-bandwidth
+    figure: on-chip reuse lets it exceed HBM peak).
     FP: every sample reads its 8-tap footprint, every ray writes one value.
     BP: every voxel-view reads its 4-tap footprint, every voxel is written."""
     if kind == "fp":
@@ -35,9 +64,134 @@ def algorithmic_bytes(kind, nvox, nviews, nrays, nsamp, s_vol, s_proj):
     return nvox * nviews * 4 * s_proj + nvox * 4
 
 
-def cpu_baseline(cfg, vol_host, angles, budget_s=12.0):
-    """FP32 OpenMP restatement (oracle/libcpucbct.so) on a bounded sample:
-    forward and back projection of the full volume for the first n views."""
+# ---------------------------------------------------------------- counters
+def read_pmc(kernel, workload, sources):
+    """Per-launch counters of a kernel from profiles/pmc_<kernel>_<workload>.json,
+    or None when absent or recorded for other kernel sources."""
+    path = os.path.join(ROOT, "profiles", "pmc_%s_%s.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("sources_sha") != sources_hash(sources):
+        return None
+    return d
+
+
+def summarize_pmc(kernel, workload, sources, dirs, out_path):
+    """Collect the per-dispatch counters of `kernel` from rocprofv3 --pmc CSV
+    output directories (one counter set per run) into one JSON summary.
+    HBM bytes: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what
+    they read for the 1 GiB device copy of the same run (--pmc-calibrate)."""
+    import csv
+    import glob
+    counters, calib, cmds = {}, {}, []
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                rows = list(csv.DictReader(f))
+            per = {}
+            for r in rows:
+                name = r["Kernel_Name"]
+                key = (name, r["Dispatch_Id"])
+                per.setdefault(key, {})
+                per[key][r["Counter_Name"]] = per[key].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            for (name, _), vals in per.items():
+                if kernel in name.split("(")[0]:
+                    for c, v in vals.items():
+                        counters.setdefault(c, []).append(v)
+                elif "copyBuffer" in name:
+                    for c, v in vals.items():
+                        calib.setdefault(c, []).append(v)
+        cmdf = os.path.join(d, "command.txt")
+        if os.path.exists(cmdf):
+            cmds.append(open(cmdf).read().strip())
+    out = {"kernel": kernel, "workload": workload, "sources_sha": sources_hash(sources),
+           "counters_per_launch": {c: sum(v) / len(v) for c, v in counters.items()},
+           "launches": {c: len(v) for c, v in counters.items()}, "commands": cmds}
+    gib_kib = float(1 << 20)
+    c = out["counters_per_launch"]
+    if "FETCH_SIZE" in c and calib.get("FETCH_SIZE"):
+        f = gib_kib / max(calib["FETCH_SIZE"])
+        out["fetch_bytes"] = c["FETCH_SIZE"] * 1024 * f
+        out["fetch_factor"] = f
+    if "WRITE_SIZE" in c and calib.get("WRITE_SIZE"):
+        f = gib_kib / max(calib["WRITE_SIZE"])
+        out["write_bytes"] = c["WRITE_SIZE"] * 1024 * f
+        out["write_factor"] = f
+    if "fetch_bytes" in out and "write_bytes" in out:
+        out["bytes_per_launch"] = out["fetch_bytes"] + out["write_bytes"]
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+def kernel_rooflines(fp_kernel, fp_ms, bp_kernel, bp_ms, workload, nvox, nviews, nrays, nsamp,
+                     s_store, filt):
+    """Roofline of each projector kernel and the ramp filter."""
+    out = {}
+    pmc = read_pmc(fp_kernel, workload, FP_SOURCES)
+    fp = {"kernel": fp_kernel, "kernel_ms": round(fp_ms, 3), "bound": "valu",
+          "peak": VALU_PEAK_TOPS, "unit": "T VALU lane-instructions/s",
+          "effective_gbs_footprint": round(footprint_bytes("fp", nvox, nviews, nrays, nsamp, s_store,
+                                                           s_store) / (fp_ms / 1e3) / 1e9, 1)}
+    if pmc:
+        c = pmc["counters_per_launch"]
+        if "SQ_INSTS_VALU" in c:
+            lane_ops = c["SQ_INSTS_VALU"] * 64
+            fp["achieved"] = round(lane_ops / (fp_ms / 1e3) / 1e12, 2)
+            fp["frac"] = round(fp["achieved"] / VALU_PEAK_TOPS, 4)
+            fp["valu_lane_ops_per_sample"] = round(lane_ops / (nrays * nsamp), 2)
+        if "bytes_per_launch" in pmc:
+            fp["traffic"] = pmc["bytes_per_launch"]
+            fp["hbm_gbs_measured"] = round(pmc["bytes_per_launch"] / (fp_ms / 1e3) / 1e9, 1)
+            fp["hbm_frac_measured"] = round(fp["hbm_gbs_measured"] / HBM_PEAK_GBS, 4)
+    out["fp"] = fp
+    pmc = read_pmc(bp_kernel, workload, BP_SOURCES)
+    bp = {"kernel": bp_kernel, "kernel_ms": round(bp_ms, 3), "bound": "hbm", "peak": HBM_PEAK_GBS,
+          "unit": "GB/s",
+          "effective_gbs_footprint": round(footprint_bytes("bp", nvox, nviews, nrays, nsamp, s_store,
+                                                           s_store) / (bp_ms / 1e3) / 1e9, 1)}
+    if pmc:
+        c = pmc["counters_per_launch"]
+        if "bytes_per_launch" in pmc:
+            bp["traffic"] = pmc["bytes_per_launch"]
+            bp["achieved"] = round(pmc["bytes_per_launch"] / (bp_ms / 1e3) / 1e9, 1)
+            bp["frac"] = round(bp["achieved"] / HBM_PEAK_GBS, 4)
+            # compulsory: the sinogram once at storage precision, the volume written once
+            bp["traffic_vs_compulsory"] = round(pmc["bytes_per_launch"] / (nrays * s_store + nvox * 4), 2)
+        if "SQ_INSTS_VALU" in c:
+            bp["valu_frac"] = round(c["SQ_INSTS_VALU"] * 64 / (bp_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)
+    out["bp"] = bp
+    if filt:
+        e1 = filt["engine1"]
+        out["filter"] = {"kernel": e1["kernel"], "kernel_ms": e1["kernel_ms"], "bound": "mfma",
+                         "peak": FP16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "achieved": e1["tflops_algorithmic"],
+                         "frac": round(e1["tflops_algorithmic"] / FP16_MFMA_PEAK_TFLOPS, 4),
+                         "issued_frac": e1["mfma_frac_of_2500_tflops"]}
+    return out
+
+
+# ------------------------------------------------------------ CPU baseline
+def host_info():
+    model = platform.processor() or ""
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(cfg, vol_host, angles, proj_f32_dev, proj_bf16_dev, bp32_dev, bp16_dev,
+                 filt_dev, ctx, budget_s=12.0):
+    """CPU baseline leg: the FP32 OpenMP restatement (oracle/libcpucbct.so)
+    and the FP64 oracle timed on this host on bounded samples of the same
+    workload, and the error of each GPU stage (FP32 and BF16 working copies)
+    against the FP64 oracle on a deterministic subsample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
@@ -45,27 +199,88 @@ def cpu_baseline(cfg, vol_host, angles, budget_s=12.0):
     g = cfg.geom()
     og = O.Geom(g.nx, g.ny, g.nz, g.lx, g.ly, g.lz, g.nu, g.nv, g.fov_u, g.fov_v, g.sad, g.sdd)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nvox = g.nx * g.ny * g.nz
 
-    def run(n):
+    def run(n, fp_fn, bp_fn, vol):
         a = angles[:n]
         t0 = time.perf_counter()
-        p = O.cpu_fp_f32(og, a, None, vol_host)
+        p = fp_fn(og, a, None, vol)
         t1 = time.perf_counter()
-        O.cpu_bp_f32(og, a, None, p)
+        bp_fn(og, a, None, p)
         t2 = time.perf_counter()
         return t1 - t0, t2 - t1
 
-    tf, tb = run(1)
+    tf, tb = run(1, O.cpu_fp_f32, O.cpu_bp_f32, vol_host)
     n = int(max(1, min(32, budget_s / max(tf + tb, 1e-3))))
     if n > 1:
-        tf, tb = run(n)
-    nvox = g.nx * g.ny * g.nz
+        tf, tb = run(n, O.cpu_fp_f32, O.cpu_bp_f32, vol_host)
     gups = 2.0 * nvox * n / (tf + tb) / 1e9
-    return {"value": round(gups, 4), "unit": "GUPS", "cores": threads, "kind": "port",
-            "sample": "FP + BP of the full 512^3 volume for the first %d of 720 views "
-                      "(FP32 OpenMP restatement; fp %.2fs, bp %.2fs)" % (n, tf, tb)}
+    res = {"value": round(gups, 4), "unit": "GUPS", "cores": threads, "kind": "port",
+           "sample": "FP + BP of the full 512^3 volume for the first %d of 720 views (FP32 "
+                     "OpenMP restatement, %d threads; fp %.2fs, bp %.2fs), extrapolated per "
+                     "voxel-view update" % (n, threads, tf, tb),
+           "host": host_info()}
+    # FP64 oracle, same kind of sample
+    v64 = vol_host.astype(np.float64)
+    tf, tb = run(1, O.fp, O.bp, v64)
+    res["fp64_oracle"] = {"value": round(2.0 * nvox / (tf + tb) / 1e9, 4), "unit": "GUPS",
+                          "sample": "FP + BP of the full volume for 1 view (fp %.2fs, bp %.2fs)"
+                                    % (tf, tb)}
+    # per-stage error against the FP64 oracle, deterministic subsample
+    import torch
+    sel = np.arange(0, len(angles), 90)
+    uu, vv = np.meshgrid(np.arange(1, g.nu, 4), np.arange(2, g.nv, 4), indexing="xy")
+    rays = np.stack([np.repeat(np.arange(len(sel)), uu.size), np.tile(uu.ravel(), len(sel)),
+                     np.tile(vv.ravel(), len(sel))], -1)
+    ref_fp = O.fp_rays(og, angles[sel], None, v64, rays)
+    sel_t = torch.from_numpy(sel).cuda()
+
+    def at_rays(p):
+        h = p[sel_t].cpu().numpy().astype(np.float64)
+        return h[rays[:, 0], rays[:, 2], rays[:, 1]]
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    vox = np.sort(np.random.default_rng(512).choice(nvox, nvox // 1000, replace=False))
+    ph = proj_f32_dev.cpu().numpy().astype(np.float64)
+    ref_bp = O.bp_voxels(og, angles, None, ph, vox)
+    del ph
+    F = np.asarray(filt_dev["F"], np.float32)
+    ph_sel = proj_f32_dev[sel_t].cpu().numpy().astype(np.float64)
+    ref_filt = O.fdk_prefilter(og, angles[sel], None, ph_sel, F)
+    st = {"subsample": "FP: views 0, 90, .., 630 x every 4th detector column and row (%d rays); "
+                       "BP: 0.1%% of the voxels (%d) over all views of the FP32 path's "
+                       "projections; filter: the same 8 views' rows" % (len(rays), len(vox)),
+          "fp_f32": rel(at_rays(proj_f32_dev), ref_fp),
+          "bp_f32": rel(bp32_dev.cpu().numpy().ravel()[vox].astype(np.float64), ref_bp)}
+    if proj_bf16_dev is not None:
+        st["fp_bf16"] = rel(at_rays(proj_bf16_dev), ref_fp)
+        st["bp_bf16"] = rel(bp16_dev.cpu().numpy().ravel()[vox].astype(np.float64), ref_bp)
+    for e in (0, 1):
+        got = filt_dev["out%d" % e][sel_t].cpu().numpy().astype(np.float64)
+        st["filter_engine%d" % e] = rel(got, ref_filt)
+    res["stage_rel_l2_vs_fp64_oracle"] = st
+    return res
 
 
+def cpu_workloads(fdk_cfg, vol_host, angles):
+    """CPU FP32 OpenMP restatement at the 128^3 FDK workload: full FP and BP."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    g = fdk_cfg.geom()
+    og = O.Geom(g.nx, g.ny, g.nz, g.lx, g.ly, g.lz, g.nu, g.nv, g.fov_u, g.fov_v, g.sad, g.sdd)
+    t0 = time.perf_counter()
+    p = O.cpu_fp_f32(og, angles, None, vol_host)
+    t1 = time.perf_counter()
+    O.cpu_bp_f32(og, angles, None, p)
+    t2 = time.perf_counter()
+    nvv = g.nx * g.ny * g.nz * len(angles)
+    return {"fp_s": round(t1 - t0, 3), "bp_s": round(t2 - t1, 3),
+            "gups_fp": round(nvv / (t1 - t0) / 1e9, 3), "gups_bp": round(nvv / (t2 - t1) / 1e9, 3)}
+
+
+# ---------------------------------------------------------------- GPU legs
 def measure_filter(ctx, g, proj, nviews, reps=3):
     """FDK pre-filter (cosine weights + ramp filter) of the whole sinogram,
     both engines.  Kernel time of the Toeplitz product from the library's own
@@ -74,12 +289,13 @@ def measure_filter(ctx, g, proj, nviews, reps=3):
     import torch
 
     from gtt import capi
-    ctx.set_filter(capi.default_filter(g.nu))
-    out = torch.empty_like(proj)
+    F = capi.default_filter(g.nu)
+    ctx.set_filter(F)
     flops_alg = 2.0 * g.nu * g.nu * g.nv * nviews
     res = {"rows": g.nv * nviews, "nu": g.nu, "flops_algorithmic": flops_alg}
-    out0 = None
+    outs = {"F": F}
     for engine in (0, 1):
+        out = torch.empty_like(proj)
         ctx.filter_engine(engine)
         ctx.fdk_prefilter(g, proj, out)          # warm-up (builds workspaces)
         torch.cuda.synchronize()
@@ -94,8 +310,7 @@ def measure_filter(ctx, g, proj, nviews, reps=3):
             kern.append(ctx.profile_read()[2])
             call.append(e0.elapsed_time(e1))
         ctx.profile(False)
-        if engine == 0:
-            out0 = out.clone()
+        outs["out%d" % engine] = out
         km = float(sorted(kern)[len(kern) // 2])
         res["engine%d" % engine] = {
             "kernel": "k_toeplitz_f32" if engine == 0 else "k_toeplitz_mfma",
@@ -103,14 +318,123 @@ def measure_filter(ctx, g, proj, nviews, reps=3):
             "tflops_algorithmic": round(flops_alg / (km / 1e3) / 1e12, 1)}
     res["engine1"]["tflops_issued"] = round(3 * flops_alg / (res["engine1"]["kernel_ms"] / 1e3) / 1e12, 1)
     res["engine1"]["mfma_frac_of_2500_tflops"] = round(res["engine1"]["tflops_issued"] / 2500.0, 4)
-    # the split-precision MFMA result against the FP32 engine (itself ~2e-7
-    # from the FP64 oracle in the parity tests), FP64 device reductions
-    res["engine1"]["rel_l2_vs_engine0"] = ctx.nrm2(out - out0) / ctx.nrm2(out0)
+    res["engine1"]["rel_l2_vs_engine0"] = ctx.nrm2(outs["out1"] - outs["out0"]) / ctx.nrm2(outs["out0"])
     ctx.filter_engine(1)
-    return res
+    return res, outs
 
 
-def serving_leg(ctx, lanes=4, n_batch=16, n_stream=24, load=0.8):
+def measured_peaks():
+    """HBM copy bandwidth (read + write bytes) and dense FP16 GEMM rate on
+    this device, for comparison with the published peaks."""
+    import torch
+    out = {}
+    n = 1 << 29   # 2 GiB of float32
+    a = torch.empty(n, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    out["hbm_copy_gbs"] = round(10 * 2 * 4 * n / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
+    del a, b
+    m = 8192
+    x = torch.randn(m, m, device="cuda", dtype=torch.float16)
+    y = torch.randn(m, m, device="cuda", dtype=torch.float16)
+    z = x @ y
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        z = x @ y
+    e1.record()
+    torch.cuda.synchronize()
+    out["fp16_gemm_tflops"] = round(10 * 2 * m ** 3 / (e0.elapsed_time(e1) / 1e3) / 1e12, 1)
+    del x, y, z
+    torch.cuda.empty_cache()
+    return out
+
+
+def workloads_leg(ctx):
+    """The 128^3 FDK workload (projector pair, FDK_reconstruct_3d latency
+    from host projections to host volume) and the 256^3 OS workload
+    (20-iteration OS-SART and osMOMSolverD + TV over 10 subsets, wall time
+    from host projections to host volume), FP32 working copies."""
+    import numpy as np
+    import torch
+
+    from gtt import operators as ops
+    from gtt import phantom as PH
+    out = {}
+    cfg = PH.FDK128
+    g = cfg.geom()
+    angles = cfg.angles()
+    vol = PH.voxelize(cfg.phantom(), cfg.vol_shape, cfg.vol_mm, device="cuda")
+    ctx.set_views(angles)
+    proj = torch.empty(g.proj_shape(len(angles)), device="cuda")
+    vout = torch.empty_like(vol)
+    for _ in range(2):
+        ctx.fp(g, vol, proj)
+        ctx.bp(g, proj, vout)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    fps, bps = [], []
+    for _ in range(5):
+        ctx.fp(g, vol, proj)
+        ctx.bp(g, proj, vout)
+        f, b, _ = ctx.profile_read()
+        fps.append(f)
+        bps.append(b)
+    ctx.profile(False)
+    nvv = g.nx * g.ny * g.nz * len(angles)
+    fpm, bpm = float(np.median(fps)), float(np.median(bps))
+    ph = proj.cpu().numpy()
+    F = np.arange(g.nu + 1, dtype=np.float32)   # Ram-Lak
+    ops.fdk_reconstruct_3d(g, angles, None, ph, filt=F)
+    lat = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        ops.fdk_reconstruct_3d(g, angles, None, ph, filt=F)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    out["fdk128"] = {"fp_ms": round(fpm, 3), "bp_ms": round(bpm, 3),
+                     "gups_fp": round(nvv / fpm / 1e6, 1), "gups_bp": round(nvv / bpm / 1e6, 1),
+                     "gups_pair": round(2 * nvv / (fpm + bpm) / 1e6, 1),
+                     "fdk_reconstruct_3d_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                                               "min": round(min(lat), 2)},
+                     "fdk_note": "host projections in, host volume out (H2D, cosine weights, "
+                                 "Ram-Lak filter, FBP, FP, LSQ rescale, D2H)"}
+    cfg = PH.OS256
+    g = cfg.geom()
+    angles = cfg.angles()
+    vol = PH.voxelize(cfg.phantom(), cfg.vol_shape, cfg.vol_mm, device="cuda")
+    ctx.set_views(angles)
+    proj = torch.empty(g.proj_shape(len(angles)), device="cuda")
+    ctx.fp(g, vol, proj)
+    clean = proj.cpu().numpy().astype(np.float64)
+    counts = np.random.default_rng(1001).poisson(1e5 * np.exp(-clean))
+    noisy = (-np.log(np.maximum(counts, 1) / 1e5)).astype(np.float32)
+    del counts, clean, proj
+    ops.os_sart(g, angles, None, noisy, 10, 1)    # warm-up
+    t0 = time.perf_counter()
+    ops.os_sart(g, angles, None, noisy, 10, 20)
+    t_sart = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ops.os_mom(g, angles, None, noisy, 10, 20, tv_weight=1e-3, tau=1e-5, reg_steps=2, nonneg=True)
+    t_mom = time.perf_counter() - t0
+    nvv = g.nx * g.ny * g.nz * len(angles)
+    out["os256"] = {"os_sart_20it_s": round(t_sart, 3), "os_mom_tv_20it_s": round(t_mom, 3),
+                    "os_sart_gups": round(2 * nvv * 21 / t_sart / 1e9, 1),
+                    "note": "10 subsets, Poisson noise at I0=1e5; wall time from host projections "
+                            "to host volume, including the normaliser passes"}
+    del vol
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving_leg(ctx, lanes=4, n_batch=64, n_stream=32, load=0.8):
     """Reconstruction latency and throughput, served (worker lanes on their
     own streams) vs plain (one lane: requests strictly one after another).
 
@@ -177,17 +501,6 @@ def serving_leg(ctx, lanes=4, n_batch=16, n_stream=24, load=0.8):
     return out
 
 
-def read_traffic(kernel, workload):
-    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
-    if not os.path.exists(path):
-        return None
-    try:
-        with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
-    except Exception:
-        return None
-
-
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,7 +512,7 @@ def main():
                          "sinogram (arithmetic is FP32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
-                    help="skip the extra bf16-storage and filter measurements")
+                    help="skip the plain FP32 path, the filter and the other workloads")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the served-vs-plain reconstruction latency measurement")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -207,7 +520,17 @@ def main():
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="with --profile-steps: first run a plain 1 GiB device copy whose "
                          "FETCH_SIZE/WRITE_SIZE scale the projector kernels' counters")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="ARG",
+                    help="KERNEL WORKLOAD {fp,bp,filter} DIR...: collect rocprofv3 --pmc CSV "
+                         "output into profiles/pmc_KERNEL_WORKLOAD.json (no GPU work)")
     args = ap.parse_args()
+
+    if args.pmc_summary:
+        kernel, workload, which = args.pmc_summary[:3]
+        src = {"fp": FP_SOURCES, "bp": BP_SOURCES, "filter": FILTER_SOURCES}[which]
+        summarize_pmc(kernel, workload, src, args.pmc_summary[3:],
+                      os.path.join(ROOT, "profiles", "pmc_%s_%s.json" % (kernel, workload)))
+        return
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path has no multi-GPU "
@@ -263,10 +586,6 @@ def main():
         ctx.storage("f32")
         return (t1 - t0) / steps * 1e3, float(np.mean(fp_ms)), float(np.mean(bp_ms)), proj, v_out
 
-    def rel_err(a, b):
-        """||a - b|| / ||b|| with FP64 device reductions."""
-        return ctx.nrm2(a - b) / ctx.nrm2(b)
-
     if args.profile_steps:
         if args.pmc_calibrate:
             src = torch.ones(1 << 28, device="cuda")   # 1 GiB
@@ -289,62 +608,71 @@ def main():
           file=sys.stderr, flush=True)
     ms_step, fp_ms, bp_ms, proj, v_out = measure(args.storage, args.steps, args.warmup)
     value = 2.0 * nvox * nviews / (ms_step / 1e3) / 1e9
+    print("bench: headline %.1f GUPS (%.2f ms per step)" % (value, ms_step), file=sys.stderr, flush=True)
 
-    s_vol = 4 if args.storage == "f32" else 2
-    kind = "fp" if fp_ms >= bp_ms else "bp"
-    kms = fp_ms if kind == "fp" else bp_ms
-    ab = algorithmic_bytes(kind, nvox, nviews, nrays, nsamp, s_vol, s_vol)
-    achieved = ab / (kms / 1e3) / 1e9
-    kname = ctx.fp_kernel(g, 0, nviews, args.storage) if kind == "fp" else "k_bp"
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic(kname, "%s_%s" % (cfg.name, args.storage)),
-                "kernel": kname, "kernel_ms": round(kms, 3), "algorithmic_bytes": ab,
-                "convention": "footprint: FP 8 taps x storage bytes per sample + 4 B per ray; "
-                              "BP 4 taps x storage bytes per voxel-view + 4 B per voxel"}
-    bp_ab = algorithmic_bytes("bp", nvox, nviews, nrays, nsamp, s_vol, s_vol)
-
+    s_store = 4 if args.storage == "f32" else 2
+    fp_kernel = ctx.fp_kernel(g, 0, nviews, args.storage)
+    bp_kernel = "k_bps" if args.storage == "f32" else "k_bp"
     extra = {"fp_ms": round(fp_ms, 3), "bp_ms": round(bp_ms, 3),
              "gups_fp": round(nvox * nviews / (fp_ms / 1e3) / 1e9, 2),
              "gups_bp": round(nvox * nviews / (bp_ms / 1e3) / 1e9, 2),
-             "fp_samples_per_s": round(nrays * nsamp / (fp_ms / 1e3), 1),
-             "bp_roofline_frac": round(bp_ab / (bp_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
-    if args.storage == "bf16":
-        # the plain FP32 path on the same inputs: its time, and the error of
-        # each BF16 stage against it (the FP32 path itself agrees with the
-        # FP64 oracle to ~1e-7 in the parity tests)
-        ms2, f2, b2, p32, v32 = measure("f32", max(1, args.steps - 1), 1)
-        g32 = 2.0 * nvox * nviews / (ms2 / 1e3) / 1e9
-        extra["plain_fp32"] = {"ms_per_step": round(ms2, 3), "fp_ms": round(f2, 3),
-                               "bp_ms": round(b2, 3), "gups": round(g32, 2),
-                               "speedup_of_bf16_storage": round(value / g32, 3)}
-        # FP stage: the step's projections; BP stage: BP of the FP32 sinogram
-        # with BF16 working copies vs without
-        ctx.storage("bf16")
-        vb = torch.empty_like(vol)
-        ctx.bp(g, p32, vb)
-        ctx.storage("f32")
-        torch.cuda.synchronize()
-        extra["stage_rel_l2_vs_fp32_path"] = {"fp": rel_err(proj, p32), "bp": rel_err(vb, v32),
-                                              "fp_then_bp": rel_err(v_out, v32)}
-        del p32, v32, vb
-
+             "fp_samples_per_s": round(nrays * nsamp / (fp_ms / 1e3), 1)}
+    p32 = v32 = filt = fouts = None
     if not args.no_variants:
-        extra["fdk_prefilter"] = measure_filter(ctx, g, proj, nviews)
+        if args.storage == "bf16":
+            # the plain FP32 path on the same inputs
+            ms2, f2, b2, p32, v32 = measure("f32", max(1, args.steps - 1), 1)
+            g32 = 2.0 * nvox * nviews / (ms2 / 1e3) / 1e9
+            extra["plain_fp32"] = {"ms_per_step": round(ms2, 3), "fp_ms": round(f2, 3),
+                                   "bp_ms": round(b2, 3), "gups": round(g32, 2),
+                                   "speedup_of_bf16_storage": round(value / g32, 3)}
+            # BF16 BP stage on the FP32 path's projections (for the oracle check)
+            ctx.storage("bf16")
+            vb = torch.empty_like(vol)
+            ctx.bp(g, p32, vb)
+            ctx.storage("f32")
+            v_out = vb
+        else:
+            p32, v32 = proj, v_out
+        filt, fouts = measure_filter(ctx, g, proj if p32 is None else p32, nviews)
+        extra["fdk_prefilter"] = filt
+        extra["measured_peaks"] = measured_peaks()
+    roof = kernel_rooflines(fp_kernel, fp_ms, bp_kernel, bp_ms, "%s_%s" % (cfg.name, args.storage),
+                            nvox, nviews, nrays, nsamp, s_store, filt)
+    roofline = dict(roof["fp"] if fp_ms >= bp_ms else roof["bp"])
+    roofline.setdefault("achieved", None)
+    roofline.setdefault("frac", None)
+    roofline.setdefault("traffic", None)
+    extra["rooflines"] = roof
+
+    cpu = None
+    if not args.no_cpu_baseline and p32 is not None:
+        vol_host = vol.cpu().numpy()
+        bf16 = args.storage == "bf16"
+        cpu = cpu_baseline(cfg, vol_host, angles, p32, proj if bf16 else None, v32,
+                           v_out if bf16 else None, fouts, ctx)
+        print("bench: cpu baseline done", file=sys.stderr, flush=True)
+    del fouts
+    torch.cuda.empty_cache()
+
+    workloads = None
+    if not args.no_variants:
+        workloads = workloads_leg(ctx)
+        if cpu is not None:
+            fc = PH.FDK128
+            fvol = PH.voxelize(fc.phantom(), fc.vol_shape, fc.vol_mm).numpy()
+            workloads["fdk128"]["cpu_fp32_openmp"] = cpu_workloads(fc, fvol, fc.angles())
+        ctx.set_views(angles, cfg.offsets())
+        print("bench: workloads done", file=sys.stderr, flush=True)
 
     latency = None
     if not args.no_serving:
-        del proj, v_out
+        del proj, v_out, p32, v32
         torch.cuda.empty_cache()
         try:
             latency = serving_leg(ctx)
         except Exception as e:  # reported in the line; the throughput figures stand
             latency = {"error": "%s: %s" % (type(e).__name__, e)}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        vol_host = vol.cpu().numpy()
-        cpu = cpu_baseline(cfg, vol_host, angles)
 
     line = {
         "metric": "fwd+back GUPS at 512^3 x 720 views",
@@ -357,17 +685,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.storage,
         "data": "synthetic (seeded ellipsoid phantom)",
         "config": {"workload": "%s: %dx%dx%d volume @ %.2f mm, %d views over 360 deg, "
                                "%dx%d detector @ %.2f mm, SAD %.0f, SDD %.0f; step = FP + BP"
                                % (cfg.name, g.nx, g.ny, g.nz, g.lx / g.nx, nviews, g.nu, g.nv,
                                   g.fov_u / g.nu, g.sad, g.sdd),
-                   "storage": "%s working copies of volume and sinogram, FP32 arithmetic"
-                              % args.storage.upper(),
+                   "storage": "%s working copies of volume and sinogram" % args.storage.upper(),
+                   "arithmetic": "FP32 (accumulation FP32; FP64 reductions)",
                    "samples_per_ray": nsamp},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "workloads": workloads,
         "latency": latency,
         "detail": extra,
     }

@@ -501,7 +501,8 @@ BPParams make_params(const cbct_geom *g, int nviews) {
 // fewer tiles than the chip holds blocks at once (1024 at four per CU): a
 // 128^3 volume makes 512 tiles of 16 z.
 constexpr int KZ = 16;
-inline int bp_kz(const cbct_geom *g) {
+inline int bp_kz(const cbct_ctx ctx, const cbct_geom *g) {
+  if (ctx->bp_kz) return ctx->bp_kz;
   const size_t tiles16 = (size_t)ceil_div(g->nx, BP_TX) * ceil_div(g->ny, BP_TY) * ceil_div(g->nz, 16);
   return tiles16 < 1024 ? 8 : KZ;
 }
@@ -535,9 +536,9 @@ int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int k
   // converts nothing: BF16 working copies use quads
   if (ctx->bp_engine == 2 && sizeof(T) == 4) {
     const size_t totalp = (size_t)P.pview * nviews;
-    int rc2 = cbct_reserve(ctx, ctx->proj_pad, totalp * sizeof(T));
+    int rc2 = cbct_reserve(ctx, ctx->ws->proj_pad, totalp * sizeof(T));
     if (rc2) return rc2;
-    T *pp2 = static_cast<T *>(ctx->proj_pad.ptr);
+    T *pp2 = static_cast<T *>(ctx->ws->proj_pad.ptr);
     hipLaunchKernelGGL((k_pad_plain<TI, T>), pad_grid(P.pnu, nviews), dim3(256), 0, s, proj, pp2,
                        g->nu, g->nv, P.pnu, P.pnv, nviews);
     CBCT_LAUNCH_CHECK(ctx, "k_pad_plain");
@@ -564,9 +565,9 @@ int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int k
   const bool quad = ctx->bp_engine == 1 || sizeof(TI) != sizeof(T);
   const unsigned pnu = (unsigned)padded_dim(g->nu), pnv = (unsigned)padded_dim(g->nv);
   const size_t total = (size_t)P.pview * nviews;
-  int rc = cbct_reserve(ctx, ctx->proj_pad, total * sizeof(T) * (quad ? 4 : 1));
+  int rc = cbct_reserve(ctx, ctx->ws->proj_pad, total * sizeof(T) * (quad ? 4 : 1));
   if (rc) return rc;
-  T *pp = static_cast<T *>(ctx->proj_pad.ptr);
+  T *pp = static_cast<T *>(ctx->ws->proj_pad.ptr);
   if (quad) {
     hipLaunchKernelGGL((k_pad_quad<TI, T>), pad_grid(pnu, nviews), dim3(256), 0, s, proj, pp, g->nu,
                        g->nv, pnu, pnv, nviews);
@@ -605,8 +606,9 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
   rc = cbct_set_device(ctx);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
+  CallScope scope(ctx, s);
   BPParams P = make_params(g, nviews);
-  const int kz = bp_kz(g);
+  const int kz = bp_kz(ctx, g);
   dim3 grid(ceil_div(g->nx, BP_TX), ceil_div(g->ny, BP_TY), ceil_div(g->nz, kz));
   if (nviews == 0) {
     // No views: the reference clears the image unless accumulating
@@ -646,6 +648,15 @@ int cbct_bp_engine(cbct_ctx ctx, int engine) {
   return CBCT_OK;
 }
 
+int cbct_bp_kz(cbct_ctx ctx, int kz) {
+  if (!ctx) return CBCT_ERR_ARG;
+  if (kz != 0 && kz != 8 && kz != KZ)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_bp_kz: %d z-voxels per thread not available (0, 8, %d)",
+                     kz, KZ);
+  ctx->bp_kz = kz;
+  return CBCT_OK;
+}
+
 int cbct_spacecarver(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *proj,
                      unsigned char *mask, float limit, void *stream) {
   if (!ctx) return CBCT_ERR_ARG;
@@ -660,6 +671,7 @@ int cbct_spacecarver(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, co
   if (rc) return rc;
   BPParams P = make_params(g, nviews);
   const size_t nvox = (size_t)g->nx * g->ny * g->nz;
+  CallScope scope(ctx, as_stream(stream));
   hipLaunchKernelGGL(k_spacecarver, stride_grid(nvox), dim3(256), 0, as_stream(stream), P, g->nu,
                      g->nv, ctx->d_views_f + view0, proj, mask, limit);
   CBCT_LAUNCH_CHECK(ctx, "k_spacecarver");

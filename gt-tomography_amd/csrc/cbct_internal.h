@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -24,6 +25,24 @@ struct DeviceBuffer {
   size_t bytes = 0;
 };
 
+// Device workspaces (grow-only until trimmed) of one context, or of several
+// that share them (cbct_ctx_share_workspace), with the event that marks the
+// end of the last call that used them.  Freed with the last context holding
+// them, after that call has finished.
+struct Workspace {
+  int device = 0;
+  DeviceBuffer vol_pad;     // padded / repacked volume for the forward projector
+  DeviceBuffer proj_pad;    // padded projections for the backprojector
+  DeviceBuffer red;         // reduction partials
+  DeviceBuffer filt_tmp;    // filter staging
+  DeviceBuffer ssim_ws;     // MS-SSIM moment fields and taps
+  hipEvent_t done = nullptr;
+  hipStream_t last = nullptr;   // stream of the last call (compared by value only)
+  bool recorded = false;
+  void free_buffers();
+  ~Workspace();
+};
+
 struct cbct_ctx_s {
   int device = 0;
   std::string err;
@@ -34,17 +53,16 @@ struct cbct_ctx_s {
   ViewD *d_views_d = nullptr;
   ViewF *d_views_f = nullptr;
   float *d_angles = nullptr;  // degrees, for the Parker weights
-  // workspaces (grow-only until trimmed)
-  DeviceBuffer vol_pad;     // padded / repacked volume for the forward projector
-  DeviceBuffer proj_pad;    // padded projections for the backprojector
-  DeviceBuffer red;         // reduction partials
-  DeviceBuffer filt_tmp;    // filter staging
-  DeviceBuffer ssim_ws;     // MS-SSIM moment fields and taps
+  std::shared_ptr<Workspace> ws;
+  // end of the last call that read this context's tables (views, filter)
+  hipEvent_t done = nullptr;
+  bool done_recorded = false;
   // filter state
   int filter_nu = 0;
-  int filter_engine = 0;
+  int filter_engine = 1;
   int fp_engine = 1;             // 1: detector-column sharing, 0: one thread per ray
   int bp_engine = 2;             // 2: LDS-staged (FP32 copies) / quad (BF16), 1: quad, 0: plain
+  int bp_kz = 0;                 // z-voxels per BP thread: 0 = chosen by size, else 8 or 16
   int fp_max_rpl = 8;            // column engine: most detector rows per lane
   int fp_view_group = 6;         // column engine: adjacent views launched together per column
   int fp_force_variant = 0;      // autotuner trials: variant and tile forced when > 0
@@ -67,6 +85,20 @@ struct cbct_ctx_s {
   hipEvent_t ev[3][2] = {};
   bool ev_recorded[3] = {false, false, false};
 };
+
+// A call that uses a context's tables or workspace: on entry the stream waits
+// for the workspace's previous call when that was made on another stream; on
+// exit the end of the call is recorded for the context and the workspace
+// (not while the stream is being captured into a graph).
+struct CallScope {
+  cbct_ctx ctx;
+  hipStream_t s;
+  bool capturing = false;
+  CallScope(cbct_ctx c, hipStream_t st);
+  ~CallScope();
+};
+// Wait (host) for the end of the last call that read the context's tables.
+int cbct_wait_tables(cbct_ctx ctx);
 
 enum { PROF_FP = 0, PROF_BP = 1, PROF_FILTER = 2 };
 // Record the start/stop event of a profiled kernel on `s` (no-op unless enabled).

@@ -72,6 +72,53 @@ int cbct_check_geom(cbct_ctx ctx, const cbct_geom *g, const char *who) {
   return CBCT_OK;
 }
 
+void Workspace::free_buffers() {
+  DeviceBuffer *bufs[] = {&vol_pad, &proj_pad, &red, &filt_tmp, &ssim_ws};
+  for (auto *b : bufs) {
+    if (b->ptr) (void)hipFree(b->ptr);
+    b->ptr = nullptr;
+    b->bytes = 0;
+  }
+}
+
+Workspace::~Workspace() {
+  (void)hipSetDevice(device);
+  if (done) {
+    if (recorded) (void)hipEventSynchronize(done);
+    (void)hipEventDestroy(done);
+  }
+  free_buffers();
+}
+
+CallScope::CallScope(cbct_ctx c, hipStream_t st) : ctx(c), s(st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    cs = hipStreamCaptureStatusNone;
+  }
+  capturing = cs != hipStreamCaptureStatusNone;
+  Workspace &w = *ctx->ws;
+  if (!capturing && w.recorded && w.last != s) (void)hipStreamWaitEvent(s, w.done, 0);
+}
+
+CallScope::~CallScope() {
+  if (capturing) return;
+  Workspace &w = *ctx->ws;
+  if (!w.done && hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess) w.done = nullptr;
+  if (!ctx->done && hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming) != hipSuccess)
+    ctx->done = nullptr;
+  if (w.done && hipEventRecord(w.done, s) == hipSuccess) {
+    w.recorded = true;
+    w.last = s;
+  }
+  if (ctx->done && hipEventRecord(ctx->done, s) == hipSuccess) ctx->done_recorded = true;
+}
+
+int cbct_wait_tables(cbct_ctx ctx) {
+  if (ctx->done_recorded) CBCT_HIP_CHECK(ctx, hipEventSynchronize(ctx->done));
+  return CBCT_OK;
+}
+
 int cbct_check_views(cbct_ctx ctx, int view0, int nviews, const char *who) {
   if (ctx->nviews == 0)
     return cbct_fail(ctx, CBCT_ERR_STATE, "Error: %s: setup not performed (no views set)", who);
@@ -99,6 +146,8 @@ int cbct_ctx_create(int device, cbct_ctx *out) {
                      device, count);
   cbct_ctx ctx = new cbct_ctx_s();
   ctx->device = device;
+  ctx->ws = std::make_shared<Workspace>();
+  ctx->ws->device = device;
   e = hipSetDevice(device);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_red), 64);
   if (e != hipSuccess) {
@@ -112,13 +161,39 @@ int cbct_ctx_create(int device, cbct_ctx *out) {
 int cbct_ctx_trim(cbct_ctx ctx) {
   if (!ctx) return CBCT_ERR_ARG;
   (void)hipSetDevice(ctx->device);
-  (void)hipDeviceSynchronize();
-  DeviceBuffer *bufs[] = {&ctx->vol_pad, &ctx->proj_pad, &ctx->red, &ctx->filt_tmp, &ctx->ssim_ws};
-  for (auto *b : bufs) {
-    if (b->ptr) (void)hipFree(b->ptr);
-    b->ptr = nullptr;
-    b->bytes = 0;
-  }
+  Workspace &w = *ctx->ws;
+  if (w.recorded) (void)hipEventSynchronize(w.done);
+  w.free_buffers();
+  return CBCT_OK;
+}
+
+int cbct_ctx_share_workspace(cbct_ctx ctx, cbct_ctx owner) {
+  if (!ctx || !owner) return CBCT_ERR_ARG;
+  if (ctx->device != owner->device)
+    return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_ctx_share_workspace: contexts on devices %d and %d",
+                     ctx->device, owner->device);
+  if (ctx->ws == owner->ws) return CBCT_OK;
+  (void)hipSetDevice(ctx->device);
+  ctx->ws = owner->ws;   // the previous workspace goes when its last holder does
+  return CBCT_OK;
+}
+
+int cbct_ctx_reset(cbct_ctx ctx) {
+  if (!ctx) return CBCT_ERR_ARG;
+  const cbct_ctx_s d;
+  ctx->filter_engine = d.filter_engine;
+  ctx->fp_engine = d.fp_engine;
+  ctx->bp_engine = d.bp_engine;
+  ctx->bp_kz = d.bp_kz;
+  ctx->fp_max_rpl = d.fp_max_rpl;
+  ctx->fp_view_group = d.fp_view_group;
+  ctx->fp_force_variant = d.fp_force_variant;
+  ctx->fp_force_rpl = d.fp_force_rpl;
+  ctx->fp_force_vg = d.fp_force_vg;
+  ctx->work_dtype = d.work_dtype;
+  ctx->profile = false;
+  for (bool &r : ctx->ev_recorded) r = false;
+  ctx->err.clear();
   return CBCT_OK;
 }
 
@@ -152,7 +227,9 @@ int cbct_profile_read(cbct_ctx ctx, double *fp_ms, double *bp_ms, double *filter
 
 int cbct_ctx_destroy(cbct_ctx ctx) {
   if (!ctx) return CBCT_OK;
-  cbct_ctx_trim(ctx);
+  (void)hipSetDevice(ctx->device);
+  (void)cbct_wait_tables(ctx);
+  if (ctx->done) (void)hipEventDestroy(ctx->done);
   for (auto &pair : ctx->ev)
     for (auto &e : pair)
       if (e) (void)hipEventDestroy(e);
@@ -174,8 +251,8 @@ const char *cbct_last_error(cbct_ctx ctx) {
 
 size_t cbct_workspace_bytes(cbct_ctx ctx) {
   if (!ctx) return 0;
-  return ctx->vol_pad.bytes + ctx->proj_pad.bytes + ctx->red.bytes + ctx->filt_tmp.bytes +
-         ctx->ssim_ws.bytes;
+  const Workspace &w = *ctx->ws;
+  return w.vol_pad.bytes + w.proj_pad.bytes + w.red.bytes + w.filt_tmp.bytes + w.ssim_ws.bytes;
 }
 
 int cbct_views_set(cbct_ctx ctx, const float *angles_deg, const float *offsets, int n) {
@@ -209,8 +286,10 @@ int cbct_views_set(cbct_ctx ctx, const float *angles_deg, const float *offsets, 
     CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_angles, sizeof(float) * n));
   }
   // Synchronous upload: callers may set views between asynchronous launches;
-  // work already enqueued that reads the old table must be complete first.
-  CBCT_HIP_CHECK(ctx, hipDeviceSynchronize());
+  // work already enqueued that reads the old table must be complete first
+  // (this context's calls only, on whatever stream they were made).
+  rc = cbct_wait_tables(ctx);
+  if (rc) return rc;
   CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_views_d, vd.data(), sizeof(ViewD) * n, hipMemcpyHostToDevice));
   CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_views_f, vf.data(), sizeof(ViewF) * n, hipMemcpyHostToDevice));
   CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_angles, angles_deg, sizeof(float) * n, hipMemcpyHostToDevice));
@@ -467,13 +546,14 @@ int cbct_reduce_f64(cbct_ctx ctx, int op, size_t n, const float *a, const float 
     return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_reduce_f64: illegal array pointer provided");
   int rc = cbct_set_device(ctx);
   if (rc) return rc;
-  rc = cbct_reserve(ctx, ctx->red, sizeof(double) * (RED_MAX_BLOCKS + 8));
+  hipStream_t s = as_stream(st);
+  CallScope scope(ctx, s);
+  rc = cbct_reserve(ctx, ctx->ws->red, sizeof(double) * (RED_MAX_BLOCKS + 8));
   if (rc) return rc;
   size_t blocks = (n + RED_BLOCK - 1) / RED_BLOCK;
   if (blocks > RED_MAX_BLOCKS) blocks = RED_MAX_BLOCKS;
-  double *partial = static_cast<double *>(ctx->red.ptr);
+  double *partial = static_cast<double *>(ctx->ws->red.ptr);
   double *res = partial + RED_MAX_BLOCKS;
-  hipStream_t s = as_stream(st);
   switch (op) {
     case CBCT_DOT:
       hipLaunchKernelGGL(k_reduce_partial<CBCT_DOT>, dim3(blocks), dim3(RED_BLOCK), 0, s, n, a, b, partial);

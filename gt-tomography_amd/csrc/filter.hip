@@ -378,9 +378,9 @@ int run_toeplitz_mfma(cbct_ctx ctx, int nu, int nv, size_t nrows, const float *i
   if (nblocks >= (1ull << 31) || mp / 4 >= (1ull << 31))
     return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_ramp_filter: too many rows");
   const size_t part = mp * kp * sizeof(_Float16);
-  rc = cbct_reserve(ctx, ctx->filt_tmp, 2 * part + mp * sizeof(float));
+  rc = cbct_reserve(ctx, ctx->ws->filt_tmp, 2 * part + mp * sizeof(float));
   if (rc) return rc;
-  char *ws = static_cast<char *>(ctx->filt_tmp.ptr);
+  char *ws = static_cast<char *>(ctx->ws->filt_tmp.ptr);
   _Float16 *xh = reinterpret_cast<_Float16 *>(ws);
   _Float16 *xl = reinterpret_cast<_Float16 *>(ws + part);
   float *rinv = reinterpret_cast<float *>(ws + 2 * part);
@@ -426,7 +426,6 @@ int ensure_cos_weights(cbct_ctx ctx, const cbct_geom *g) {
   ctx->d_cosw = nullptr;
   ctx->cosw_valid = false;
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_cosw, sizeof(float) * w.size()));
-  CBCT_HIP_CHECK(ctx, hipDeviceSynchronize());
   CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_cosw, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
   ctx->cosw_geom = *g;
   ctx->cosw_valid = true;
@@ -438,11 +437,11 @@ int run_toeplitz(cbct_ctx ctx, int nu, size_t nrows, const float *in, float *out
   const float *src = in;
   if (in == out) {
     // the tiled kernel reads whole rows while writing: stage the input
-    int rc = cbct_reserve(ctx, ctx->filt_tmp, sizeof(float) * nrows * nu);
+    int rc = cbct_reserve(ctx, ctx->ws->filt_tmp, sizeof(float) * nrows * nu);
     if (rc) return rc;
-    CBCT_HIP_CHECK(ctx, hipMemcpyAsync(ctx->filt_tmp.ptr, in, sizeof(float) * nrows * nu,
+    CBCT_HIP_CHECK(ctx, hipMemcpyAsync(ctx->ws->filt_tmp.ptr, in, sizeof(float) * nrows * nu,
                                        hipMemcpyDeviceToDevice, s));
-    src = static_cast<const float *>(ctx->filt_tmp.ptr);
+    src = static_cast<const float *>(ctx->ws->filt_tmp.ptr);
   }
   const size_t ytiles = (nrows + TT - 1) / TT;
   if (ytiles > 0x7fffffffull)
@@ -497,7 +496,9 @@ int cbct_filter_set(cbct_ctx ctx, int nu, const float *F) {
   std::vector<double> h = spatial_kernel(nu, F);
   std::vector<float> hm(2 * (size_t)nu - 1);
   for (int d = -(nu - 1); d <= nu - 1; ++d) hm[d + nu - 1] = (float)h[d < 0 ? -d : d];
-  CBCT_HIP_CHECK(ctx, hipDeviceSynchronize());
+  // calls still reading the old tables finish first
+  rc = cbct_wait_tables(ctx);
+  if (rc) return rc;
   if (ctx->d_h) (void)hipFree(ctx->d_h);
   if (ctx->d_thi) (void)hipFree(ctx->d_thi);
   if (ctx->d_tlo) (void)hipFree(ctx->d_tlo);
@@ -528,6 +529,7 @@ int cbct_ramp_filter(cbct_ctx ctx, int nu, size_t nrows, const float *in, float 
     return cbct_fail(ctx, CBCT_ERR_ARG, "cbct_ramp_filter: filter was set for nu=%d, called with nu=%d", ctx->filter_nu, nu);
   int rc = cbct_set_device(ctx);
   if (rc) return rc;
+  CallScope scope(ctx, as_stream(stream));
   if (ctx->filter_engine == 1)
     return run_toeplitz_mfma(ctx, nu, 1, nrows, in, out, nullptr, nullptr, 0, 0.0f, 0.0f,
                              as_stream(stream));
@@ -550,9 +552,10 @@ int cbct_fdk_prefilter(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, 
   if (nviews == 0) return CBCT_OK;
   rc = cbct_set_device(ctx);
   if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  CallScope scope(ctx, s);
   rc = ensure_cos_weights(ctx, g);
   if (rc) return rc;
-  hipStream_t s = as_stream(stream);
   const size_t total = (size_t)g->nu * g->nv * nviews;
   const float *d_angles = ctx->d_angles + view0;
   const float delta = std::atan(g->fov_u / (2.0f * g->sdd));
@@ -565,9 +568,9 @@ int cbct_fdk_prefilter(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, 
     if (rc) return rc;
   } else {
   // stage 1: weights into the staging buffer
-  rc = cbct_reserve(ctx, ctx->filt_tmp, sizeof(float) * total);
+  rc = cbct_reserve(ctx, ctx->ws->filt_tmp, sizeof(float) * total);
   if (rc) return rc;
-  float *tmp = static_cast<float *>(ctx->filt_tmp.ptr);
+  float *tmp = static_cast<float *>(ctx->ws->filt_tmp.ptr);
   hipLaunchKernelGGL(k_cos_parker, stride_grid(total), dim3(256), 0, s, in, tmp, ctx->d_cosw, g->nu,
                      g->nv, nviews, d_angles, short_scan ? 1 : 0, delta, tan_delta);
   CBCT_LAUNCH_CHECK(ctx, "k_cos_parker");
@@ -600,6 +603,7 @@ int cbct_offset_weights(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews,
   rc = cbct_set_device(ctx);
   if (rc) return rc;
   const size_t total = (size_t)g->nu * g->nv * nviews;
+  CallScope scope(ctx, as_stream(stream));
   hipLaunchKernelGGL(k_offset_weights, stride_grid(total), dim3(256), 0, as_stream(stream), proj,
                      g->nu, g->nv, nviews, ctx->d_views_f + view0, g->fov_u, sqrt_variant ? 1 : 0);
   CBCT_LAUNCH_CHECK(ctx, "k_offset_weights");
@@ -619,11 +623,12 @@ int cbct_parker_weights(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews,
   rc = cbct_set_device(ctx);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
+  CallScope scope(ctx, s);
   // unit "cosine weights": reuse k_cos_parker with a ones plane
   const size_t plane = (size_t)g->nu * g->nv;
-  rc = cbct_reserve(ctx, ctx->filt_tmp, sizeof(float) * plane);
+  rc = cbct_reserve(ctx, ctx->ws->filt_tmp, sizeof(float) * plane);
   if (rc) return rc;
-  float *ones = static_cast<float *>(ctx->filt_tmp.ptr);
+  float *ones = static_cast<float *>(ctx->ws->filt_tmp.ptr);
   rc = cbct_fill(ctx, plane, ones, 1.0f, stream);
   if (rc) return rc;
   const float *d_angles = ctx->d_angles + view0;

@@ -1046,9 +1046,9 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
   const FPChoice choice = select_variant(ctx, g, P, view0, nviews, esz, spp);
   if (choice.variant == 0) {
     const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
-    int rc = cbct_reserve(ctx, ctx->vol_pad, npad * esz);
+    int rc = cbct_reserve(ctx, ctx->ws->vol_pad, npad * esz);
     if (rc) return rc;
-    T *vp = static_cast<T *>(ctx->vol_pad.ptr);
+    T *vp = static_cast<T *>(ctx->ws->vol_pad.ptr);
     hipLaunchKernelGGL((k_pad_volume<TI, T>), stride_grid(npad), dim3(256), 0, s, vol, vp, g->nx, g->ny,
                        g->nz, P.px, (unsigned)padded_dim(g->ny), (unsigned)padded_dim(g->nz));
     CBCT_LAUNCH_CHECK(ctx, "k_pad_volume");
@@ -1064,9 +1064,9 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
   const size_t npad = (size_t)P.pzx * py;
   // z-line chunk loads may run up to 127 elements past a column's end (those
   // values are never used); the slack keeps such reads inside this allocation
-  int rc = cbct_reserve(ctx, ctx->vol_pad, (npad + 1024) * esz);
+  int rc = cbct_reserve(ctx, ctx->ws->vol_pad, (npad + 1024) * esz);
   if (rc) return rc;
-  T *vt = static_cast<T *>(ctx->vol_pad.ptr);
+  T *vt = static_cast<T *>(ctx->ws->vol_pad.ptr);
   dim3 tgrid(ceil_div(P.pz, 32), ceil_div(P.px, 32), (unsigned)py);
   hipLaunchKernelGGL((k_pad_transpose<TI, T>), tgrid, dim3(32, 8), 0, s, vol, vt, g->nx, g->ny, g->nz,
                      P.px, P.pz);
@@ -1148,6 +1148,7 @@ int cbct_fp_autotune(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, in
     return cbct_fail(ctx, CBCT_ERR_NOMEM, "%s: scratch allocation failed", who);
   }
   hipStream_t s = as_stream(stream);
+  CallScope scope(ctx, s);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
@@ -1254,6 +1255,7 @@ int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
   rc = cbct_set_device(ctx);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
+  CallScope scope(ctx, s);
   if (dtype == CBCT_BF16)
     return run_fp<unsigned short, unsigned short>(ctx, g, P, view0, nviews,
                                                   static_cast<const unsigned short *>(vol), proj,
@@ -1282,11 +1284,12 @@ int cbct_fp_transpose(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, c
   if (rc) return rc;
   rc = cbct_set_device(ctx);
   if (rc) return rc;
-  const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
-  rc = cbct_reserve(ctx, ctx->vol_pad, npad * sizeof(float));
-  if (rc) return rc;
   hipStream_t s = as_stream(stream);
-  float *acc = static_cast<float *>(ctx->vol_pad.ptr);
+  CallScope scope(ctx, s);
+  const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
+  rc = cbct_reserve(ctx, ctx->ws->vol_pad, npad * sizeof(float));
+  if (rc) return rc;
+  float *acc = static_cast<float *>(ctx->ws->vol_pad.ptr);
   hipLaunchKernelGGL(k_zero, stride_grid(npad), dim3(256), 0, s, npad, acc);
   CBCT_LAUNCH_CHECK(ctx, "k_zero");
   if (nviews > 0) {

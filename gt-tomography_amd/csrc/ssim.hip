@@ -210,12 +210,13 @@ extern "C" int cbct_mssim(cbct_ctx ctx, int nx, int ny, int nz, const float *ima
   }
   int rc = cbct_set_device(ctx);
   if (rc) return rc;
+  CallScope scope(ctx, as_stream(st));
   const size_t n = (size_t)nx * ny * nz;
   const size_t fields = 11 * n;   // F[5], G[5], S
   const size_t bytes = sizeof(float) * (fields + taps.size() + 2 * MM_MAX_BLOCKS + 64);
-  rc = cbct_reserve(ctx, ctx->ssim_ws, bytes);
+  rc = cbct_reserve(ctx, ctx->ws->ssim_ws, bytes);
   if (rc) return rc;
-  float *F = static_cast<float *>(ctx->ssim_ws.ptr);
+  float *F = static_cast<float *>(ctx->ws->ssim_ws.ptr);
   float *G = F + 5 * n;
   float *S = G + 5 * n;
   float *d_taps = S + n;

@@ -92,6 +92,8 @@ def load():
         "cbct_last_error": (ctypes.c_char_p, [P]),
         "cbct_workspace_bytes": (S, [P]),
         "cbct_ctx_trim": (I, [P]),
+        "cbct_ctx_share_workspace": (I, [P, P]),
+        "cbct_ctx_reset": (I, [P]),
         "cbct_profile_enable": (I, [P, I]),
         "cbct_profile_read": (I, [P, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_double),
@@ -108,6 +110,7 @@ def load():
         "cbct_fp_tile": (I, [P, I]),
         "cbct_fp_view_group": (I, [P, I]),
         "cbct_bp_engine": (I, [P, I]),
+        "cbct_bp_kz": (I, [P, I]),
         "cbct_storage": (I, [P, I]),
         "cbct_spacecarver": (I, [P, G, I, I, P, P, F, P]),
         "cbct_apply_mask": (I, [P, S, P, P, P]),
@@ -215,6 +218,14 @@ class Context:
     def trim(self):
         self._check(self.lib.cbct_ctx_trim(self.h))
 
+    def share_workspace(self, owner: "Context"):
+        """Use owner's scratch from now on (one host thread at a time)."""
+        self._check(self.lib.cbct_ctx_share_workspace(self.h, owner.h))
+
+    def reset(self):
+        """Settings back to their defaults; views, filter and scratch kept."""
+        self._check(self.lib.cbct_ctx_reset(self.h))
+
     def profile(self, on=True):
         """Bracket the FP / BP / filter kernels with HIP events on their stream."""
         self._check(self.lib.cbct_profile_enable(self.h, int(on)))
@@ -285,6 +296,10 @@ class Context:
         """2 = default (LDS-staged kernel for FP32 copies, else as 1), 1 = quad-texel
         projection copy, 0 = plain padded copy."""
         self._check(self.lib.cbct_bp_engine(self.h, int(engine)))
+
+    def bp_kz(self, kz):
+        """Backprojector z-voxels per thread: 0 = by size (default), 8 or 16."""
+        self._check(self.lib.cbct_bp_kz(self.h, int(kz)))
 
     def storage(self, dtype):
         """Working-copy storage of the projectors for FP32 arrays: 'f32' or 'bf16'."""

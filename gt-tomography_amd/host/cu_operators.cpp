@@ -49,11 +49,11 @@ void upload_views(cbct_ctx ctx, const std::vector<float> &a, const std::vector<f
 
 // ---------------------------------------------------------------------------
 cuConebeamProjectionOperator::cuConebeamProjectionOperator() {
-  check(cbct_ctx_create(current_device(), &ctx_), nullptr);
+  ctx_ = acquire_context();
   check(cbct_storage(ctx_, default_storage()), ctx_);
 }
 
-cuConebeamProjectionOperator::~cuConebeamProjectionOperator() { cbct_ctx_destroy(ctx_); }
+cuConebeamProjectionOperator::~cuConebeamProjectionOperator() { release_context(ctx_); }
 
 cbct_geom cuConebeamProjectionOperator::geom_for(size_t nx, size_t ny, size_t nz, size_t nu,
                                                  size_t nv) const {

@@ -29,9 +29,7 @@ const float kPi = 3.14159265358979f;
 
 struct CtxGuard {
   cbct_ctx c = nullptr;
-  ~CtxGuard() {
-    if (c) cbct_ctx_destroy(c);
-  }
+  ~CtxGuard() { release_context(c); }
 };
 
 struct ScanFlags {
@@ -66,7 +64,7 @@ ScanFlags operator_setup(CBCT_acquisition &acquisition) {
 // when none is given).
 void make_context(CtxGuard &ctx, const std::vector<float> &ta, const std::vector<float> &to, size_t nu,
                   std::shared_ptr<std::vector<float>> frequency_filter, const char *who) {
-  check(cbct_ctx_create(current_device(), &ctx.c), nullptr);
+  ctx.c = acquire_context();
   check(cbct_storage(ctx.c, default_storage()), ctx.c);
   check(cbct_views_set(ctx.c, ta.data(), to.data(), (int)ta.size()), ctx.c);
   std::vector<float> F;

@@ -19,11 +19,11 @@ const float kPi = 3.14159265358979f;
 
 
 hoCuConebeamProjectionOperator::hoCuConebeamProjectionOperator() {
-  check(cbct_ctx_create(current_device(), &ctx_), nullptr);
+  ctx_ = acquire_context();
   check(cbct_storage(ctx_, default_storage()), ctx_);
 }
 
-hoCuConebeamProjectionOperator::~hoCuConebeamProjectionOperator() { cbct_ctx_destroy(ctx_); }
+hoCuConebeamProjectionOperator::~hoCuConebeamProjectionOperator() { release_context(ctx_); }
 
 void hoCuConebeamProjectionOperator::set_binning(std::shared_ptr<CBCT_binning> b) {
   binning_ = std::move(b);

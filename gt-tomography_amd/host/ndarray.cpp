@@ -48,8 +48,51 @@ void set_default_graphs(bool on) { t_graphs = on; }
 bool default_graphs() { return t_graphs; }
 long long graph_launches() { return t_graph_launches; }
 void set_current_stream(void *stream) { t_stream = static_cast<hipStream_t>(stream); }
+namespace {
+// Pooled projector contexts of the calling thread, by device; and every
+// context the thread handed out (to recognise returns from other threads).
+thread_local std::map<int, std::vector<cbct_ctx>> t_pool;
+thread_local std::map<cbct_ctx, int> t_pool_owned;   // context -> its device
+}  // namespace
+
+cbct_ctx acquire_context() {
+  const int dev = current_device();
+  auto &free_list = t_pool[dev];
+  if (!free_list.empty()) {
+    cbct_ctx c = free_list.back();
+    free_list.pop_back();
+    check(cbct_ctx_reset(c), c);
+    return c;
+  }
+  cbct_ctx owner = device_context();
+  cbct_ctx c = nullptr;
+  check(cbct_ctx_create(dev, &c), nullptr);
+  int rc = cbct_ctx_share_workspace(c, owner);
+  if (rc) {
+    std::string msg = cbct_last_error(c);
+    cbct_ctx_destroy(c);
+    throw std::runtime_error(msg);
+  }
+  t_pool_owned[c] = dev;
+  return c;
+}
+
+void release_context(cbct_ctx c) {
+  if (!c) return;
+  auto it = t_pool_owned.find(c);
+  if (it == t_pool_owned.end()) {   // taken on another thread
+    cbct_ctx_destroy(c);
+    return;
+  }
+  t_pool[it->second].push_back(c);
+}
+
 void release_thread_resources() {
   trim_device_cache();
+  for (auto &kv : t_pool)
+    for (cbct_ctx c : kv.second) cbct_ctx_destroy(c);
+  t_pool.clear();
+  t_pool_owned.clear();
   for (auto &kv : t_ctx) cbct_ctx_destroy(kv.second);
   t_ctx.clear();
 }
@@ -84,6 +127,21 @@ void trim_device_cache() {
   t_cached_bytes = 0;
 }
 
+size_t retained_device_bytes() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return t_cached_bytes;
+  auto it = t_ctx.find(dev);
+  return t_cached_bytes + (it == t_ctx.end() ? 0 : cbct_workspace_bytes(it->second));
+}
+
+void trim_thread_memory() {
+  trim_device_cache();
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  auto it = t_ctx.find(dev);
+  if (it != t_ctx.end()) (void)cbct_ctx_trim(it->second);
+}
+
 void device_malloc(void **p, size_t bytes) {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
@@ -98,9 +156,9 @@ void device_malloc(void **p, size_t bytes) {
     }
   }
   hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess && !t_free_blocks.empty()) {
+  if (e != hipSuccess && retained_device_bytes() > 0 && !t_capture_freed) {
     (void)hipGetLastError();
-    trim_device_cache();
+    trim_thread_memory();
     e = hipMalloc(p, bytes);
   }
   hip_check(e, "hipMalloc");

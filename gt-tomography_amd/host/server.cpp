@@ -13,8 +13,14 @@
 //
 // Admission: a request's HBM footprint is estimated from its sizes at submit;
 // the head of the FIFO starts when a lane is free and the footprints of the
-// running requests plus its own fit the budget (no overtaking, so the order
-// of starts is the order of arrival).
+// running requests, the memory idle lanes keep, and its own fit the budget
+// (no overtaking, so the order of starts is the order of arrival).
+//
+// A lane keeps its device memory between requests (its pooled projector
+// contexts, their shared scratch, the freed blocks of its arrays): the next
+// request of similar size reuses it with no allocation and no device-wide
+// synchronisation.  What a lane keeps is counted against the budget while it
+// is idle; when the head of the queue does not fit, idle lanes free theirs.
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -56,12 +62,24 @@ double ms(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-// Device bytes a request may hold at once (volume-sized and projection-sized
-// arrays of the operators, solvers and padded kernel copies), with slack.
+// Device bytes a request may hold at once, in FP32 elements (BF16 working
+// copies need less), with slack:
+//  OS-SART: the iterate, the update image and S column normalisers
+//    (S + 2 volumes); the uploaded projections and their subset-ordered copy,
+//    the row normalisers and the residual (4 sinograms); the scratch the
+//    subset operators share: the padded volume copy (up to 1.3 volumes for
+//    small grids) and the padded copy of one subset's projections (up to 4x
+//    a subset sinogram as quad texels).
+//  FDK: the image and the padded volume copy (2.3 volumes); the projections,
+//    the filtered projections, A f, the filter staging and the padded copy of
+//    all views (up to 4x as quad texels): 8 sinograms.
 size_t footprint_bytes(const gtt_request &r) {
   const size_t nvox = (size_t)r.geom.nx * r.geom.ny * r.geom.nz;
   const size_t nproj = (size_t)(r.geom.nu + 3) * (r.geom.nv + 3) * (size_t)r.nviews;
-  return 4 * (6 * nvox + 8 * nproj) + (64u << 20);
+  const size_t slack = size_t(64) << 20;
+  if (r.kind == GTT_REQ_FDK) return 4 * (nvox * 23 / 10 + 8 * nproj) + slack;
+  const size_t S = r.subsets > 0 ? (size_t)r.subsets : 1;
+  return 4 * (nvox * (10 * S + 33) / 10 + 4 * nproj + 4 * ((nproj + S - 1) / S)) + slack;
 }
 
 // The request's storage type is the lane's default while its operators are
@@ -109,6 +127,7 @@ struct gtt_server_s {
   bool autotune = false;
   size_t budget = 0;
   size_t in_use = 0;
+  size_t kept = 0;                 // memory idle lanes keep (sum over lanes)
   int running = 0;
   bool stop = false;
   long long next_id = 1;
@@ -120,7 +139,7 @@ struct gtt_server_s {
   std::string err;
 
   bool head_fits() const {
-    return !queue.empty() && in_use + queue.front()->footprint <= budget;
+    return !queue.empty() && in_use + kept + queue.front()->footprint <= budget;
   }
 
   void lane_main() {
@@ -130,19 +149,40 @@ struct gtt_server_s {
         hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
       init_err = "gtt_server: cannot create the lane's HIP stream";
     set_current_stream(st);
+    size_t my_kept = 0;   // this lane's share of `kept`
     for (;;) {
       std::shared_ptr<Job> j;
+      bool trim_first = false;
       {
         std::unique_lock<std::mutex> lk(mu);
-        // queued work drains before a stop takes effect
-        cv.wait(lk, [&] { return (stop && queue.empty()) || head_fits(); });
-        if (!head_fits()) break;
+        // queued work drains before a stop takes effect; a lane that keeps
+        // memory wakes to free it when the head does not fit otherwise
+        cv.wait(lk, [&] {
+          return (stop && queue.empty()) || head_fits() || (!queue.empty() && my_kept > 0);
+        });
+        if (!head_fits()) {
+          if (queue.empty()) break;
+          lk.unlock();
+          trim_thread_memory();
+          lk.lock();
+          kept -= my_kept;
+          my_kept = 0;
+          cv.notify_all();
+          continue;
+        }
         j = queue.front();
         queue.pop_front();
+        // the memory this lane keeps becomes part of the request's own; more
+        // than the request's footprint is freed first, so that the request
+        // holds no more than its footprint says
+        trim_first = my_kept > j->footprint;
+        kept -= my_kept;
+        my_kept = 0;
         in_use += j->footprint;
         ++running;
         j->t_start = Clock::now();
       }
+      if (trim_first) trim_thread_memory();
       int status = 0;
       std::string msg;
       if (!init_err.empty()) {
@@ -156,9 +196,9 @@ struct gtt_server_s {
           status = -1;
           msg = e.what();
         }
-        // the request's cached device blocks go back before its footprint does
-        trim_device_cache();
+        if (status) trim_thread_memory();   // after a failure, start clean
       }
+      const size_t keep = init_err.empty() ? retained_device_bytes() : 0;
       {
         std::lock_guard<std::mutex> lk(mu);
         j->t_end = Clock::now();
@@ -166,11 +206,17 @@ struct gtt_server_s {
         j->message = msg;
         j->done = true;
         in_use -= j->footprint;
+        my_kept = keep;
+        kept += keep;
         --running;
       }
       cv.notify_all();
     }
     release_thread_resources();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      kept -= my_kept;
+    }
     if (st) (void)hipStreamDestroy(st);
   }
 };

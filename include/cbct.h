@@ -18,6 +18,12 @@
  *    layer rethrows these as std::runtime_error with the reference's text.
  *  - A context is single-threaded; distinct contexts may be used concurrently.
  *    There is no global mutable state.
+ *  - Calls on one context may use different streams: a call that uses the
+ *    context's scratch first makes its stream wait for the previous such call
+ *    when that was enqueued on another stream, so the scratch is never
+ *    overwritten while a kernel still reads it.  cbct_views_set and
+ *    cbct_filter_set wait (on the host) for this context's earlier calls
+ *    only, not for the whole device.
  */
 #ifndef CBCT_H
 #define CBCT_H
@@ -65,8 +71,21 @@ size_t cbct_workspace_bytes(cbct_ctx ctx);
  * durations in ms (negative if that kernel has not run since enabling). */
 int cbct_profile_enable(cbct_ctx ctx, int on);
 int cbct_profile_read(cbct_ctx ctx, double *fp_ms, double *bp_ms, double *filter_ms);
-/* Frees the context's scratch (it regrows on demand). */
+/* Frees the context's scratch (it regrows on demand), after the last call
+ * that used it has finished. */
 int cbct_ctx_trim(cbct_ctx ctx);
+/* ctx drops its own scratch and uses owner's from now on (same device).  The
+ * scratch lives until the last context holding it is destroyed.  Contexts
+ * that share scratch are used from one host thread at a time; their calls
+ * are ordered as calls on one context are (above).  Lets the many per-subset
+ * operator contexts of one reconstruction hold one padded volume and one
+ * padded sinogram copy between them instead of one each.  No reference
+ * counterpart (the reference allocates these per call). */
+int cbct_ctx_share_workspace(cbct_ctx ctx, cbct_ctx owner);
+/* Settings back to their defaults (engines, tile, view group, z-depth,
+ * storage type, profiling); the view table, filter and scratch are kept.
+ * For callers that reuse contexts.  No reference counterpart. */
+int cbct_ctx_reset(cbct_ctx ctx);
 
 /* View table: n views, angles in degrees, offsets (u, v) in mm interleaved
  * (offsets_host may be NULL = all zero).  Replaces the per-call
@@ -151,6 +170,11 @@ int cbct_fp_view_group(cbct_ctx ctx, int views);
  * bitwise-identical volumes.  No reference counterpart (the reference samples
  * a layered texture). */
 int cbct_bp_engine(cbct_ctx ctx, int engine);
+
+/* Backprojector: z-voxels per thread, 0 = chosen by size (16, or 8 when 16
+ * would make fewer than 1024 tiles), or 8 / 16 forced (tests run both at
+ * every size).  Results do not depend on it.  No reference counterpart. */
+int cbct_bp_kz(cbct_ctx ctx, int kz);
 
 /* Storage type of the projectors' working copies (the zero-bordered volume
  * copy the forward projector gathers from, the projection copy the

@@ -30,6 +30,17 @@ void *current_stream();
 void set_current_stream(void *stream);
 // Destroy the calling thread's contexts (worker threads, before they exit).
 void release_thread_resources();
+// A projector context for an operator or driver: taken from the calling
+// thread's pool for the current device (settings reset, view table and
+// filter to be set by the caller), created when the pool is empty.  Every
+// context of a thread's pool shares the scratch of device_context(), so the
+// operators of one reconstruction hold one padded volume and one padded
+// sinogram copy between them; contexts and scratch are kept across
+// reconstructions (no device synchronisation per operator).
+cbct_ctx acquire_context();
+// Returns a context to the pool of the thread that took it; a context given
+// back on another thread is destroyed instead.
+void release_context(cbct_ctx ctx);
 // Storage type (CBCT_F32 or CBCT_BF16) of the projectors' working copies for
 // projection operators the calling thread creates from now on (default FP32).
 void set_default_storage(int dtype);
@@ -54,6 +65,11 @@ void device_free(void *p);
 // Frees the device blocks the calling thread keeps for reuse (device_free
 // caches blocks per thread, device and stream).
 void trim_device_cache();
+// Bytes of device memory the calling thread keeps between uses: its cached
+// blocks and the scratch of its contexts on the current device.
+size_t retained_device_bytes();
+// Frees both (after the thread's last calls that used them have finished).
+void trim_thread_memory();
 // Executable HIP graph of the device work a function queues on the calling
 // thread's current stream (which must not be the default stream: it cannot
 // be captured).  The function runs once, during the capture, queueing

@@ -144,6 +144,30 @@ void orc_fp(const orc_geom *g, const float *angles_deg, const float *offsets, in
   }
 }
 
+/* orc_fp evaluated on a list of rays only (view k, column i, row j packed as
+ * rays[3r..3r+2]): the same ray setup and sample sum as orc_fp
+ * (conebeam_projection.cu:412-528), so out[r] equals orc_fp's value at that
+ * detector pixel.  Lets full-size geometries be checked on a subsample. */
+void orc_fp_rays(const orc_geom *g, const float *angles_deg, const float *offsets,
+                 const double *vol, float samples_per_pixel, size_t nrays, const int *rays,
+                 double *out) {
+  int n = orc_num_samples(samples_per_pixel, g->nx);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long r = 0; r < (long)nrays; ++r) {
+    int k = rays[3 * r], i = rays[3 * r + 1], j = rays[3 * r + 2];
+    double p0[3], q[3], step;
+    fp_ray(g, angles_deg[k], offsets[2 * k], offsets[2 * k + 1], i, j, n, p0, q, &step);
+    double acc = 0.0;
+    for (int m = 0; m < n; ++m) {
+      double tx = (p0[0] + q[0] * m) * g->nx - 0.5;
+      double ty = (p0[1] + q[1] * m) * g->ny - 0.5;
+      double tz = (p0[2] + q[2] * m) * g->nz - 0.5;
+      acc += tri_sample(vol, g->nx, g->ny, g->nz, tx, ty, tz);
+    }
+    out[r] = acc * step;
+  }
+}
+
 /* Exact transpose of orc_fp (ray-driven scatter).  vol += FP^T(proj).
  * Not a reference function: it exists so that <FP x, y> = <x, FP^T y> can be
  * checked (the reference FP/BP pair is not adjoint). */
@@ -243,6 +267,33 @@ void orc_bp(const orc_geom *g, const float *angles_deg, const float *offsets, in
       }
 }
 
+/* orc_bp evaluated at a list of voxels only (linear indices x + nx*(y + ny*z)):
+ * the same projection, weight and sum over views as orc_bp
+ * (conebeam_projection.cu:922-1037), so out[r] equals orc_bp's value there. */
+void orc_bp_voxels(const orc_geom *g, const float *angles_deg, const float *offsets, int nviews,
+                   const double *proj, int fbp, size_t nvox, const long long *voxels,
+                   double *out) {
+  const size_t npix = (size_t)g->nu * g->nv;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long r = 0; r < (long)nvox; ++r) {
+    long long idx = voxels[r];
+    int x = (int)(idx % g->nx);
+    int y = (int)((idx / g->nx) % g->ny);
+    int z = (int)(idx / ((long long)g->nx * g->ny));
+    double pos[3] = { ((x + 0.5) / g->nx - 0.5) * g->lx,
+                      ((y + 0.5) / g->ny - 0.5) * g->ly,
+                      ((z + 0.5) / g->nz - 0.5) * g->lz };
+    double acc = 0.0;
+    for (int k = 0; k < nviews; ++k) {
+      double tu, tv;
+      bp_project(g, angles_deg[k], offsets[2 * k], offsets[2 * k + 1], pos, &tu, &tv);
+      double w = fbp ? fbp_weight(g, angles_deg[k], pos) : 1.0;
+      acc += w * bi_sample(proj + npix * k, g->nu, g->nv, tu, tv);
+    }
+    out[r] = acc;
+  }
+}
+
 /* Exact transpose of orc_bp (voxel-driven splat into the projections). */
 void orc_bp_transpose(const orc_geom *g, const float *angles_deg, const float *offsets,
                       int nviews, const double *vol, double *proj, int fbp, int accumulate) {
@@ -320,17 +371,21 @@ void orc_filter_kernel(int nu, const double *F, double *h) {
 
 /* Filter every detector row in place: y[i] = sum_j h[|i-j|] x[j]. */
 void orc_filter_rows(int nu, size_t nrows, const double *h, double *data) {
-  double *tmp = (double *)malloc(sizeof(double) * nu);
-  for (size_t r = 0; r < nrows; ++r) {
-    double *row = data + r * nu;
-    for (int i = 0; i < nu; ++i) {
-      double acc = 0.0;
-      for (int j = 0; j < nu; ++j) acc += h[abs(i - j)] * row[j];
-      tmp[i] = acc;
+#pragma omp parallel
+  {
+    double *tmp = (double *)malloc(sizeof(double) * nu);
+#pragma omp for schedule(static)
+    for (long r = 0; r < (long)nrows; ++r) {
+      double *row = data + (size_t)r * nu;
+      for (int i = 0; i < nu; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < nu; ++j) acc += h[abs(i - j)] * row[j];
+        tmp[i] = acc;
+      }
+      memcpy(row, tmp, sizeof(double) * nu);
     }
-    memcpy(row, tmp, sizeof(double) * nu);
+    free(tmp);
   }
-  free(tmp);
 }
 
 /* Parker-type short-scan weights (Wesarg q=0.1): conebeam_projection.cu:149-194. */

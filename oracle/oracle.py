@@ -95,6 +95,8 @@ def _load():
         _lib.orc_fp_transpose.argtypes = [g, f, f, i, d, d, ctypes.c_float, i]
         _lib.orc_bp.argtypes = [g, f, f, i, d, d, i, i]
         _lib.orc_bp_transpose.argtypes = [g, f, f, i, d, d, i, i]
+        _lib.orc_fp_rays.argtypes = [g, f, f, d, ctypes.c_float, ctypes.c_size_t, P(i), d]
+        _lib.orc_bp_voxels.argtypes = [g, f, f, i, d, i, ctypes.c_size_t, P(ctypes.c_longlong), d]
         _lib.orc_cosine_weights.argtypes = [g, d]
         _lib.orc_guo_filter.argtypes = [i, d]
         _lib.orc_filter_kernel.argtypes = [i, d, d]
@@ -168,6 +170,23 @@ def fp(geom: Geom, angles, offsets, vol, samples_per_pixel=1.5, out=None, accumu
     return out
 
 
+def fp_rays(geom: Geom, angles, offsets, vol, rays, samples_per_pixel=1.5):
+    """:func:`fp` at a list of rays only: ``rays`` is (nrays, 3) of (view, u, v)
+    indices; returns the nrays projection values fp() gives there."""
+    lib = _load()
+    a, o = _views(angles, offsets)
+    v = np.ascontiguousarray(vol, dtype=np.float64).reshape(geom.vol_shape)
+    r = np.ascontiguousarray(rays, dtype=np.int32).reshape(-1, 3)
+    if r.size and (r[:, 0].min() < 0 or r[:, 0].max() >= a.size or r[:, 1].min() < 0
+                   or r[:, 1].max() >= geom.nu or r[:, 2].min() < 0 or r[:, 2].max() >= geom.nv):
+        raise ValueError("ray index out of range")
+    out = np.zeros(len(r), np.float64)
+    g = geom.c()
+    lib.orc_fp_rays(ctypes.byref(g), _fp_(a), _fp_(o), _dp(v), ctypes.c_float(samples_per_pixel),
+                    len(r), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(out))
+    return out
+
+
 def fp_transpose(geom: Geom, angles, offsets, proj, samples_per_pixel=1.5, out=None,
                  accumulate=False):
     """Exact transpose of :func:`fp` (scatter)."""
@@ -193,6 +212,22 @@ def bp(geom: Geom, angles, offsets, proj, fbp=False, out=None, accumulate=False)
     g = geom.c()
     lib.orc_bp(ctypes.byref(g), _fp_(a), _fp_(o), a.size, _dp(p), _dp(out), int(fbp),
                int(accumulate))
+    return out
+
+
+def bp_voxels(geom: Geom, angles, offsets, proj, voxels, fbp=False):
+    """:func:`bp` at a list of voxels only (linear indices x + nx*(y + ny*z));
+    returns the values bp() gives there."""
+    lib = _load()
+    a, o = _views(angles, offsets)
+    p = np.ascontiguousarray(proj, dtype=np.float64).reshape(geom.proj_shape(a.size))
+    vx = np.ascontiguousarray(voxels, dtype=np.int64).ravel()
+    if vx.size and (vx.min() < 0 or vx.max() >= geom.nx * geom.ny * geom.nz):
+        raise ValueError("voxel index out of range")
+    out = np.zeros(vx.size, np.float64)
+    g = geom.c()
+    lib.orc_bp_voxels(ctypes.byref(g), _fp_(a), _fp_(o), a.size, _dp(p), int(fbp), vx.size,
+                      vx.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _dp(out))
     return out
 
 

@@ -80,15 +80,23 @@ def test_libgtt_exports_every_declared_symbol():
 
 def test_request_footprint_and_server_without_gpu():
     """Host-only parts of the server API: the footprint estimate (volume- and
-    padded-projection-sized arrays, plus slack) and a clean error from
-    gtt_server_create when no device can be selected."""
+    padded-projection-sized arrays, by request kind and subset count, plus
+    slack) and a clean error from gtt_server_create when no device can be
+    selected."""
     from gtt import serving as S
     spec = S.ScanSpec(n=40, nviews=30, nu=64, nv=48, subsets=5, kind=S.REQ_OS_SART, seed=1)
     g = spec.geom()
     req = S.Request(kind=S.REQ_OS_SART, geom=g, angles=spec.angles(),
                     projections=np.zeros(g.proj_shape(30), np.float32))
     nvox, nproj = 40 ** 3, (64 + 3) * (48 + 3) * 30
-    assert req.footprint() == 4 * (6 * nvox + 8 * nproj) + (64 << 20)
+    # OS-SART: S + 2 volumes, the shared padded volume copy (<= 1.3), four
+    # sinograms and one subset's quad-texel copy; FDK: 2.3 volumes, 8 sinograms
+    for S_ in (1, 5, 12):
+        req.subsets = S_
+        want = 4 * (nvox * (10 * S_ + 33) // 10 + 4 * nproj + 4 * (-(-nproj // S_))) + (64 << 20)
+        assert req.footprint() == want
+    req.kind = S.REQ_FDK
+    assert req.footprint() == 4 * (nvox * 23 // 10 + 8 * nproj) + (64 << 20)
     try:
         import torch
         if torch.cuda.is_available():

@@ -178,18 +178,25 @@ def test_fp_split_launches(ctx):
     assert float(outs[1].abs().sum()) > 0.0
 
 
+@pytest.mark.parametrize("kz", [0, 16])
 @pytest.mark.parametrize("fbp", [False, True])
 @pytest.mark.parametrize("name", list(CASES))
-def test_bp_parity(ctx, name, fbp):
+def test_bp_parity(ctx, name, fbp, kz):
+    """kz 0: z-voxels per thread chosen by size (8 at these sizes); 16: the
+    instantiation large volumes use, forced."""
     g, angles, offsets = make_case(name)
-    print_case("bp", name, fbp, g)
+    print_case("bp", name, fbp, kz, g)
     proj = np.random.default_rng(2).random(g.proj_shape(len(angles)))
     ref = O.bp(to_oracle(g), angles, offsets, proj, fbp=fbp)
     ctx.set_views(angles, offsets)
     out = torch.empty(g.vol_shape, device="cuda")
-    ctx.bp(g, dev(proj), out, fbp=fbp)
+    ctx.bp_kz(kz)
+    try:
+        ctx.bp(g, dev(proj), out, fbp=fbp)
+    finally:
+        ctx.bp_kz(0)
     err = rel_l2(host(out), ref)
-    print("bp rel-L2", name, fbp, err)
+    print("bp rel-L2", name, fbp, kz, err)
     assert err <= TOL
 
 
@@ -197,25 +204,31 @@ def test_bp_parity(ctx, name, fbp):
 @pytest.mark.parametrize("name", list(CASES))
 def test_bp_engines_bitwise_identical(ctx, name, dtype):
     """Quad-texel and plain padded projection copies, and the LDS-staged
-    kernel, give identical bits (FBP and plain, FP32 and BF16 storage,
-    accumulating into a volume)."""
+    kernel, give identical bits at 8 and at 16 z-voxels per thread (FBP and
+    plain, FP32 and BF16 storage, accumulating into a volume).  The two
+    z-depths differ in rounding: each thread steps v from its first voxel."""
     g, angles, offsets = make_case(name)
     print_case("bp engines", name, dtype, g)
     proj = np.random.default_rng(4).random(g.proj_shape(len(angles)))
     tdt = torch.float32 if dtype == "f32" else torch.bfloat16
     ctx.set_views(angles, offsets)
     start = torch.from_numpy(np.random.default_rng(6).random(g.vol_shape).astype(np.float32)).cuda()
-    for fbp in (False, True):
-        outs = []
-        for e in (0, 1, 2):
-            ctx.bp_engine(e)
-            out = start.clone()
-            ctx.bp(g, dev(proj, tdt), out, fbp=fbp, accumulate=True)
-            torch.cuda.synchronize()
-            outs.append(out)
+    try:
+        for fbp in (False, True):
+            for kz in (8, 16):
+                ctx.bp_kz(kz)
+                outs = []
+                for e in (0, 1, 2):
+                    ctx.bp_engine(e)
+                    out = start.clone()
+                    ctx.bp(g, dev(proj, tdt), out, fbp=fbp, accumulate=True)
+                    torch.cuda.synchronize()
+                    outs.append(out)
+                for i in range(1, len(outs)):
+                    assert torch.equal(outs[0], outs[i]), (fbp, kz, i)
+    finally:
         ctx.bp_engine(2)
-        assert torch.equal(outs[0], outs[1]), fbp
-        assert torch.equal(outs[0], outs[2]), fbp
+        ctx.bp_kz(0)
 
 
 @pytest.mark.parametrize("name", list(CASES))

@@ -160,3 +160,38 @@ def test_served_autotuned_equals_alone(small_requests):
         capi.fp_autotune_clear()
     for r, out, want in zip(small_requests, res, ref):
         assert np.array_equal(out["volume"], want), (r.kind, r.geom.nx)
+
+
+@pytest.mark.parametrize("kind", [S.REQ_OS_SART, S.REQ_FDK])
+def test_footprint_covers_measured_peak(ctx, kind):
+    """The admission estimate covers the device memory a request really
+    takes: one request (12 subsets for OS-SART) on a fresh one-lane server,
+    device free memory polled while it runs; the peak use (free memory
+    before minus the lowest seen) stays within gtt_request_footprint."""
+    import threading
+    spec = S.ScanSpec(n=200, nviews=240, nu=400, nv=300, subsets=12, kind=kind, seed=77)
+    req = S.make_request(spec, ctx, iterations=2)
+    est = req.footprint()
+    print_case("footprint vs measured peak, kind", kind, "estimate", est)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    low = [free0]
+    stop = threading.Event()
+
+    def poll():
+        while not stop.is_set():
+            low[0] = min(low[0], torch.cuda.mem_get_info()[0])
+
+    t = threading.Thread(target=poll)
+    t.start()
+    try:
+        with S.Server(lanes=1) as srv:
+            rid = srv.submit(req)
+            out = srv.wait(rid)
+    finally:
+        stop.set()
+        t.join()
+    peak = free0 - low[0]
+    print("kind", kind, "peak", peak, "estimate", est, "ratio %.2f" % (peak / est))
+    assert np.all(np.isfinite(out["volume"]))
+    assert 0 < peak <= est

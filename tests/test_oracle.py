@@ -210,6 +210,28 @@ def test_fp_ray_missing_box_is_not_guarded():
     assert p[0, 2, 4] > 0             # central ray hits
 
 
+def test_fp_rays_and_bp_voxels_are_pointwise_fp_and_bp():
+    """The subsample forms used at full-size geometries give exactly the
+    full projector's values at their rays / voxels (offsets and FBP too)."""
+    g = small_geom(nx=9, ny=7, nz=8, nu=13, nv=11)
+    angles = np.array([3.0, 47.0, 181.5, 300.0], np.float32)
+    offs = np.array([[0.5, -0.3], [0.0, 0.0], [-1.1, 0.7], [2.0, 0.1]], np.float32)
+    vol = rand_vol(g, 8)
+    full = O.fp(g, angles, offs, vol)
+    rng = np.random.default_rng(2)
+    rays = np.stack([rng.integers(0, 4, 50), rng.integers(0, g.nu, 50), rng.integers(0, g.nv, 50)], -1)
+    got = O.fp_rays(g, angles, offs, vol, rays)
+    assert np.array_equal(got, full[rays[:, 0], rays[:, 2], rays[:, 1]])
+    for fbp in (False, True):
+        bfull = O.bp(g, angles, offs, full, fbp=fbp)
+        vox = rng.choice(g.nx * g.ny * g.nz, 40, replace=False)
+        assert np.array_equal(O.bp_voxels(g, angles, offs, full, vox, fbp=fbp), bfull.ravel()[vox])
+    with pytest.raises(ValueError):
+        O.fp_rays(g, angles, offs, vol, [[4, 0, 0]])
+    with pytest.raises(ValueError):
+        O.bp_voxels(g, angles, offs, full, [g.nx * g.ny * g.nz])
+
+
 def test_adjoint_fp_transpose():
     g = small_geom(nx=7, ny=6, nz=5, nu=9, nv=8)
     rng = np.random.default_rng(7)
@@ -563,6 +585,106 @@ def test_os_mom_tv_reduces_noise_energy():
     z1 = O.os_mom(op, pb, 3, tv_weight=1e-3, tau0=1e-3, reg_steps=2, nonneg=True)
     tv = lambda v: np.abs(np.diff(v, axis=0)).sum() + np.abs(np.diff(v, axis=1)).sum() + np.abs(np.diff(v, axis=2)).sum()
     assert tv(z1) < tv(z0)
+
+
+class _IdentityOp:
+    """One-subset identity 'projector' (codomain = domain), so that the
+    preconditioner is P = 1 and a solver step can be followed by hand."""
+    nsubsets = 1
+
+    def __init__(self, shape):
+        self.shape = shape
+
+    def domain_shape(self):
+        return self.shape
+
+    def projection_subsets(self, arr):
+        return [arr]
+
+    def mult_M(self, x, out, accumulate=False):
+        out[...] = (out if accumulate else 0.0) + x
+
+    mult_MH = mult_M
+
+    def mult_M_subset(self, x, out, subset, accumulate=False):
+        self.mult_M(x, out, accumulate)
+
+    mult_MH_subset = mult_M_subset
+
+
+def test_os_mom_tv_schedule_by_hand():
+    """osMOMSolverD::denoise schedule (solvers/osMOMSolverD.h:205-227) with
+    every constant written out: one subset, identity projector (P = 1),
+    TV weight 0.5 (so avg_lambda = 0.5 and the normalised TV weight is 1),
+    tau0 = 1e-3, reg_steps = 2, start x0.
+      gamma~ = 0.35 / 0.5 / (N / asum(P)) = 0.7
+      sigma0 = 1 / (tau0 * L^2), L = 4         = 62.5
+      theta1 = 1 / sqrt(1 + 2 * 0.7 * 1e-3)    = 0.999300734143549
+      tau1   = tau0 * theta1                   = 0.000999300734143549
+      sigma1 = sigma0 / theta1                 = 62.543734698209384
+    The gradient step takes z to b (S = beta = 1, A = I, P = 1); each TV
+    step is x <- (x - tau g + (tau/0.5) b) / (1 + tau/0.5) with
+    g = D^T proj(sigma D x) (updateG, :233-247); the momentum factor of the
+    first subset is 0, so the solver returns x2.  Dual fields stay inside
+    the unit ball, so sigma enters linearly: sigma0 = 1/(4 tau0), a theta of
+    1, or gamma~ without the 0.35 all change the result by far more than
+    the tolerance."""
+    rng = np.random.default_rng(21)
+    shape = (4, 4, 4)
+    x0 = 0.004 * rng.random(shape)
+    b = 0.004 * rng.random(shape)
+    tau0, tau1 = 1e-3, 0.000999300734143549
+    sigma0, sigma1 = 62.5, 62.543734698209384
+    lam = 0.5
+
+    def dual_norm_max(x, sigma):
+        d = np.stack([np.roll(x, -1, axis=2) - x, np.roll(x, -1, axis=1) - x,
+                      np.roll(x, -1, axis=0) - x]) * sigma
+        return np.sqrt((d ** 2).sum(0)).max()
+
+    assert dual_norm_max(x0, sigma0) < 1.0
+    g1 = np_tv(x0, sigma0, 0.0, 1.0)
+    x1 = (x0 - tau0 * g1 + (tau0 / lam) * b) / (1.0 + tau0 / lam)
+    assert dual_norm_max(x1, sigma1) < 1.0
+    g2 = np_tv(x1, sigma1, 0.0, 1.0)
+    x2 = (x1 - tau1 * g2 + (tau1 / lam) * b) / (1.0 + tau1 / lam)
+
+    z = O.os_mom(_IdentityOp(shape), b, 1, tv_weight=lam, tau0=tau0, reg_steps=2, x0=x0)
+    assert np.allclose(z, x2, rtol=1e-13, atol=1e-17)
+    # the schedule matters at this tolerance: a wrong sigma0 (1/(4 tau0))
+    # moves the result by orders of magnitude more than it
+    g1w = np_tv(x0, 1.0 / (4 * tau0), 0.0, 1.0)
+    x1w = (x0 - tau0 * g1w + (tau0 / lam) * b) / (1.0 + tau0 / lam)
+    assert np.abs(x1w - x1).max() > 1e3 * 1e-13 * np.abs(x1).max()
+
+
+def test_os_mom_second_subset_momentum_with_tv():
+    """Two iterations of one subset on the identity projector: the second
+    subset step starts from z1 = x2 of the first (factor 0), then
+    t2 = (1 + sqrt(1 + 4 t1^2)) / 2 and z = x' (1 + c) - c x_prev with
+    c = (t1 - 1) / t2 (solvers/osMOMSolverD.h:139-181), with the TV schedule
+    restarting at tau0 in every denoise call."""
+    rng = np.random.default_rng(5)
+    shape = (4, 4, 4)
+    b = 0.004 * rng.random(shape)
+    tau0, tau1 = 1e-3, 0.000999300734143549
+    sigma0, sigma1 = 62.5, 62.543734698209384
+    lam = 0.5
+
+    def denoise(x, s):
+        x = (x - tau0 * np_tv(x, sigma0, 0.0, 1.0) + (tau0 / lam) * s) / (1.0 + tau0 / lam)
+        return (x - tau1 * np_tv(x, sigma1, 0.0, 1.0) + (tau1 / lam) * s) / (1.0 + tau1 / lam)
+
+    t1 = 0.5 * (1 + math.sqrt(5.0))
+    xa = denoise(np.zeros(shape), b)          # subset step 1 from x = z = 0: z <- b
+    z1 = xa                                   # factor (1 - 1)/t1 = 0
+    t2 = 0.5 * (1 + math.sqrt(1 + 4 * t1 * t1))
+    s2 = z1 - (z1 - b)                        # gradient step on z = z1 (A = I): b
+    xb = denoise(z1, s2)                      # x = z1 at the second step
+    c = (t1 - 1) / t2
+    z2 = xb * (1 + c) - c * xa
+    z = O.os_mom(_IdentityOp(shape), b, 2, tv_weight=lam, tau0=tau0, reg_steps=2)
+    assert np.allclose(z, z2, rtol=1e-13, atol=1e-17)
 
 
 def test_fdk_scale_cancels_in_driver():
