@@ -78,6 +78,16 @@ def read_pmc(kernel, workload, sources):
     return d
 
 
+def kernel_matches(kernel, name):
+    """Whether a profiler kernel name (demangled, or an Itanium-mangled one
+    such as _ZN12_GLOBAL__N_115k_toeplitz_mfmaE...) is `kernel` itself, not
+    a longer name that starts with it."""
+    import re
+    if name.startswith("_Z"):
+        return ("%d%s" % (len(kernel), kernel)) in name
+    return re.search(r"(?<![A-Za-z0-9_])%s(?=[<(])" % re.escape(kernel), name) is not None
+
+
 def summarize_pmc(kernel, workload, sources, dirs, out_path):
     """Collect the per-dispatch counters of `kernel` from rocprofv3 --pmc CSV
     output directories (one counter set per run) into one JSON summary.
@@ -97,7 +107,7 @@ def summarize_pmc(kernel, workload, sources, dirs, out_path):
                 per.setdefault(key, {})
                 per[key][r["Counter_Name"]] = per[key].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
             for (name, _), vals in per.items():
-                if kernel in name.split("(")[0]:
+                if kernel_matches(kernel, name):
                     for c, v in vals.items():
                         counters.setdefault(c, []).append(v)
                 elif "copyBuffer" in name:
