@@ -327,6 +327,14 @@ class Context:
                                               _ptr(mask), float(limit), _stream(stream)))
         return mask
 
+    def apply_mask(self, image, mask, stream=None):
+        """image *= mask (uint8 device tensor of the image's size)."""
+        if mask.numel() != image.numel():
+            raise ValueError("apply_mask: mask and image sizes differ")
+        self._check(self.lib.cbct_apply_mask(self.h, image.numel(), _ptr(image), _ptr(mask),
+                                             _stream(stream)))
+        return image
+
     # ------------------------------------------------------------- FDK
     def set_filter(self, F):
         import numpy as np

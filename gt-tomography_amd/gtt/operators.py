@@ -243,3 +243,141 @@ class CBSubsetOperator:
     def mult_MH(self, proj, vol, subset=-1, accumulate=False):
         _check(self.L.gtt_subset_op_mult_MH(self.h, ctypes.c_void_p(proj.data_ptr()),
                                             ctypes.c_void_p(vol.data_ptr()), int(subset), int(accumulate)))
+
+
+def _bind_extra(L):
+    P, I, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    G = ctypes.POINTER(capi.Geom)
+    IP = ctypes.POINTER(ctypes.c_int)
+    for name, args in {
+        "gtt_cu_op_create": [ctypes.POINTER(P)], "gtt_cu_op_destroy": [P],
+        "gtt_cu_op_setup": [P, G, P, P, I, P, P, I, I, I],
+        "gtt_cu_op_mult_M": [P, P, P, I], "gtt_cu_op_mult_MH": [P, P, P, I],
+        "gtt_cu_op_calculate_mask": [P, P, F, P], "gtt_cu_op_set_mask": [P, P],
+        "gtt_cu_op_flags": [P, IP, IP],
+        "gtt_ho_op_create": [ctypes.POINTER(P)], "gtt_ho_op_destroy": [P],
+        "gtt_ho_op_setup": [P, G, P, P, I, P, P, I, I, I, P, F],
+        "gtt_ho_op_mult_M": [P, P, P, I], "gtt_ho_op_mult_MH": [P, P, P, I],
+        "gtt_ho_op_flags": [P, IP, IP],
+        "gtt_write_real": [ctypes.c_char_p, I, ctypes.POINTER(ctypes.c_size_t), P],
+        "gtt_read_real_header": [ctypes.c_char_p, IP, ctypes.POINTER(ctypes.c_size_t)],
+        "gtt_read_real": [ctypes.c_char_p, P, ctypes.c_size_t],
+    }.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = I
+    return L
+
+
+def _dptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class _OpBase:
+    _prefix = ""
+
+    def __init__(self):
+        self.L = _bind_extra(load())
+        h = ctypes.c_void_p()
+        _check(getattr(self.L, self._prefix + "create")(ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                getattr(self.L, self._prefix + "destroy")(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def flags(self):
+        """(short_scan, use_offset_correction) as setup decided them."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(getattr(self.L, self._prefix + "flags")(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return bool(a.value), bool(b.value)
+
+
+class CuConebeamProjectionOperator(_OpBase):
+    """cuConebeamProjectionOperator on torch device tensors (default stream).
+    use_offset_correction: None = decided by setup, else forced."""
+    _prefix = "gtt_cu_op_"
+
+    def setup(self, g: capi.Geom, angles, offsets=None, bins=None, transform_angles=True,
+              use_offset_correction=None):
+        a, o = _views(angles, offsets)
+        idx, sizes, nb = _bins(bins)
+        uoc = -1 if use_offset_correction is None else int(bool(use_offset_correction))
+        _check(self.L.gtt_cu_op_setup(self.h, ctypes.byref(g), _p(a), _p(o), a.size, _p(idx),
+                                      _p(sizes), nb, int(bool(transform_angles)), uoc))
+        self.g = g
+        return self
+
+    def mult_M(self, vol, proj, accumulate=False):
+        _check(self.L.gtt_cu_op_mult_M(self.h, _dptr(vol), _dptr(proj), int(accumulate)))
+
+    def mult_MH(self, proj, vol, accumulate=False):
+        _check(self.L.gtt_cu_op_mult_MH(self.h, _dptr(proj), _dptr(vol), int(accumulate)))
+
+    def calculate_mask(self, proj, limit, mask):
+        """mask: uint8 device tensor of the domain's shape."""
+        _check(self.L.gtt_cu_op_calculate_mask(self.h, _dptr(proj), float(limit), _dptr(mask)))
+        return mask
+
+    def set_mask(self, mask):
+        _check(self.L.gtt_cu_op_set_mask(self.h, _dptr(mask)))
+
+
+class HoCuConebeamProjectionOperator(_OpBase):
+    """hoCuConebeamProjectionOperator on host numpy arrays."""
+    _prefix = "gtt_ho_op_"
+
+    def setup(self, g: capi.Geom, angles, offsets=None, bins=None, use_fbp=False,
+              use_offset_correction=None, filt=None, samples_per_pixel=1.5):
+        a, o = _views(angles, offsets)
+        idx, sizes, nb = _bins(bins)
+        uoc = -1 if use_offset_correction is None else int(bool(use_offset_correction))
+        F = None if filt is None else np.ascontiguousarray(filt, np.float32)
+        if F is not None and F.size != g.nu + 1:
+            raise ValueError("the frequency filter holds nu+1 taps")
+        _check(self.L.gtt_ho_op_setup(self.h, ctypes.byref(g), _p(a), _p(o), a.size, _p(idx),
+                                      _p(sizes), nb, int(bool(use_fbp)), uoc, _p(F),
+                                      float(samples_per_pixel)))
+        self.g, self.nviews, self.nbins = g, a.size, max(nb, 1)
+        return self
+
+    def _vol_shape(self):
+        return _vol_shape(self.g, self.nbins)
+
+    def mult_M(self, image, proj, accumulate=False):
+        x = np.ascontiguousarray(image, np.float32)
+        if x.shape != self._vol_shape() or proj.dtype != np.float32 or not proj.flags.c_contiguous:
+            raise ValueError("mult_M: image %s and float32 projections expected" % (self._vol_shape(),))
+        _check(self.L.gtt_ho_op_mult_M(self.h, _p(x), _p(proj), int(accumulate)))
+        return proj
+
+    def mult_MH(self, proj, image, accumulate=False):
+        y = np.ascontiguousarray(proj, np.float32)
+        if image.shape != self._vol_shape() or image.dtype != np.float32 or not image.flags.c_contiguous:
+            raise ValueError("mult_MH: float32 image %s expected" % (self._vol_shape(),))
+        _check(self.L.gtt_ho_op_mult_MH(self.h, _p(y), _p(image), int(accumulate)))
+        return image
+
+
+def write_real(path, array):
+    """Write a numpy array as a .real file (C order (..., ny, nx) = sizes
+    listed first-index-fastest in the file)."""
+    L = _bind_extra(load())
+    a = np.ascontiguousarray(array, np.float32)
+    dims = (ctypes.c_size_t * a.ndim)(*a.shape[::-1])
+    _check(L.gtt_write_real(os.fsencode(path), a.ndim, dims, _p(a)))
+
+
+def read_real(path):
+    L = _bind_extra(load())
+    nd = ctypes.c_int()
+    dims = (ctypes.c_size_t * 16)()
+    _check(L.gtt_read_real_header(os.fsencode(path), ctypes.byref(nd), dims))
+    shape = tuple(dims[i] for i in range(nd.value))[::-1]
+    out = np.empty(shape, np.float32)
+    _check(L.gtt_read_real(os.fsencode(path), _p(out), out.size))
+    return out

@@ -75,6 +75,26 @@ struct gtt_subset_op_s {
   std::shared_ptr<CBSubsetOperator> op;
 };
 
+struct gtt_cu_op_s {
+  std::shared_ptr<cuConebeamProjectionOperator> op;
+  std::vector<size_t> dom;          // nx, ny, nz[, nbins]
+  std::vector<size_t> codom;        // nu, nv, views in the bins
+};
+
+struct gtt_ho_op_s {
+  std::shared_ptr<hoCuConebeamProjectionOperator> op;
+  std::vector<size_t> dom;
+  std::vector<size_t> codom;        // nu, nv, nviews (acquisition order)
+};
+
+namespace {
+size_t views_in_bins(const CBCT_binning &b) {
+  size_t n = 0;
+  for (unsigned i = 0; i < b.get_number_of_bins(); ++i) n += b.bin(i).size();
+  return n;
+}
+}  // namespace
+
 extern "C" {
 
 const char *gtt_last_error(void) { return g_err.c_str(); }
@@ -298,6 +318,175 @@ int gtt_subset_op_mult_MH(gtt_subset_op op, float *proj, float *vol, int subset,
       cuNDArray<float> y(*op->op->get_codomain_dimensions(subset), proj);
       op->op->mult_MH(&y, &x, subset, accumulate != 0);
     }
+  });
+}
+
+int gtt_cu_op_create(gtt_cu_op *out) {
+  return guarded([&] {
+    if (!out) throw std::runtime_error("gtt_cu_op_create: null output");
+    auto *h = new gtt_cu_op_s();
+    h->op = std::make_shared<cuConebeamProjectionOperator>();
+    *out = h;
+  });
+}
+
+int gtt_cu_op_destroy(gtt_cu_op op) {
+  return guarded([&] { delete op; });
+}
+
+int gtt_cu_op_setup(gtt_cu_op op, const cbct_geom *g, const float *angles, const float *offsets,
+                    int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
+                    int transform_angles, int use_offset_correction) {
+  return guarded([&] {
+    if (!op) throw std::runtime_error("null operator");
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, nullptr);
+    op->op = std::make_shared<cuConebeamProjectionOperator>();
+    if (use_offset_correction >= 0) op->op->set_use_offset_correction(use_offset_correction != 0);
+    op->op->setup(s.acq, s.binning, s.dims_mm, transform_angles != 0);
+    op->op->set_domain_dimensions(&s.vol_dims);
+    op->dom = s.vol_dims;
+    op->codom = {(size_t)g->nu, (size_t)g->nv, views_in_bins(*s.binning)};
+    op->op->set_codomain_dimensions(&op->codom);
+  });
+}
+
+int gtt_cu_op_mult_M(gtt_cu_op op, const float *vol, float *proj, int accumulate) {
+  return guarded([&] {
+    if (!op || op->dom.empty()) throw std::runtime_error("Error: cuConebeamProjectionOperator::mult_M: setup not performed");
+    cuNDArray<float> x(op->dom, const_cast<float *>(vol));
+    cuNDArray<float> y(op->codom, proj);
+    op->op->mult_M(&x, &y, accumulate != 0);
+  });
+}
+
+int gtt_cu_op_mult_MH(gtt_cu_op op, const float *proj, float *vol, int accumulate) {
+  return guarded([&] {
+    if (!op || op->dom.empty()) throw std::runtime_error("Error: cuConebeamProjectionOperator::mult_MH: setup not performed");
+    cuNDArray<float> y(op->codom, const_cast<float *>(proj));
+    cuNDArray<float> x(op->dom, vol);
+    op->op->mult_MH(&y, &x, accumulate != 0);
+  });
+}
+
+int gtt_cu_op_calculate_mask(gtt_cu_op op, const float *proj, float limit, unsigned char *mask) {
+  return guarded([&] {
+    if (!op || op->dom.empty()) throw std::runtime_error("calculate_mask: setup not performed");
+    if (!proj || !mask) throw std::runtime_error("calculate_mask: illegal array pointer provided");
+    cuNDArray<float> y(op->codom, const_cast<float *>(proj));
+    auto m = op->op->calculate_mask(&y, limit);
+    copy_d2d(mask, m->get_data_ptr(), m->get_number_of_bytes());
+  });
+}
+
+int gtt_cu_op_set_mask(gtt_cu_op op, const unsigned char *mask) {
+  return guarded([&] {
+    if (!op || op->dom.empty()) throw std::runtime_error("set_mask: setup not performed");
+    if (!mask) {
+      op->op->set_mask(nullptr);
+      return;
+    }
+    auto m = std::make_shared<cuNDArray<unsigned char>>(op->dom);
+    copy_d2d(m->get_data_ptr(), mask, m->get_number_of_bytes());
+    op->op->set_mask(m);
+  });
+}
+
+int gtt_cu_op_flags(gtt_cu_op op, int *short_scan, int *use_offset_correction) {
+  return guarded([&] {
+    if (!op) throw std::runtime_error("null operator");
+    if (short_scan) *short_scan = op->op->is_short_scan() ? 1 : 0;
+    if (use_offset_correction) *use_offset_correction = op->op->get_use_offset_correction() ? 1 : 0;
+  });
+}
+
+int gtt_ho_op_create(gtt_ho_op *out) {
+  return guarded([&] {
+    if (!out) throw std::runtime_error("gtt_ho_op_create: null output");
+    auto *h = new gtt_ho_op_s();
+    h->op = std::make_shared<hoCuConebeamProjectionOperator>();
+    *out = h;
+  });
+}
+
+int gtt_ho_op_destroy(gtt_ho_op op) {
+  return guarded([&] { delete op; });
+}
+
+int gtt_ho_op_setup(gtt_ho_op op, const cbct_geom *g, const float *angles, const float *offsets,
+                    int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
+                    int use_fbp, int use_offset_correction, const float *filter, float spp) {
+  return guarded([&] {
+    if (!op) throw std::runtime_error("null operator");
+    Scan s = make_scan(g, angles, offsets, nviews, bin_index, bin_sizes, nbins, nullptr);
+    op->op = std::make_shared<hoCuConebeamProjectionOperator>();
+    if (use_offset_correction >= 0) op->op->set_use_offset_correction(use_offset_correction != 0);
+    op->op->set_use_filtered_backprojection(use_fbp != 0);
+    if (spp > 0) op->op->set_num_samples_per_pixel(spp);
+    if (filter)
+      op->op->set_frequency_filter(std::make_shared<std::vector<float>>(filter, filter + g->nu + 1));
+    op->op->setup(s.acq, s.binning, s.dims_mm, true);
+    op->op->set_domain_dimensions(&s.vol_dims);
+    op->dom = s.vol_dims;
+    op->codom = {(size_t)g->nu, (size_t)g->nv, (size_t)nviews};
+    op->op->set_codomain_dimensions(&op->codom);
+  });
+}
+
+int gtt_ho_op_mult_M(gtt_ho_op op, const float *image, float *proj, int accumulate) {
+  return guarded([&] {
+    if (!op || op->dom.empty()) throw std::runtime_error("Error: hoCuConebeamProjectionOperator::mult_M: setup not performed");
+    if (!image || !proj) throw std::runtime_error("Error: hoCuConebeamProjectionOperator::mult_M: illegal array pointer provided");
+    hoCuNDArray<float> x(op->dom, const_cast<float *>(image));
+    hoCuNDArray<float> y(op->codom, proj);
+    op->op->mult_M(&x, &y, accumulate != 0);
+  });
+}
+
+int gtt_ho_op_mult_MH(gtt_ho_op op, const float *proj, float *image, int accumulate) {
+  return guarded([&] {
+    if (!op || op->dom.empty()) throw std::runtime_error("Error: hoCuConebeamProjectionOperator::mult_MH: setup not performed");
+    if (!image || !proj) throw std::runtime_error("Error: hoCuConebeamProjectionOperator::mult_MH:: illegal array pointer provided");
+    hoCuNDArray<float> y(op->codom, const_cast<float *>(proj));
+    hoCuNDArray<float> x(op->dom, image);
+    op->op->mult_MH(&y, &x, accumulate != 0);
+  });
+}
+
+int gtt_ho_op_flags(gtt_ho_op op, int *short_scan, int *use_offset_correction) {
+  return guarded([&] {
+    if (!op) throw std::runtime_error("null operator");
+    if (short_scan) *short_scan = op->op->is_short_scan() ? 1 : 0;
+    if (use_offset_correction) *use_offset_correction = op->op->get_use_offset_correction() ? 1 : 0;
+  });
+}
+
+int gtt_write_real(const char *path, int ndims, const size_t *dims, const float *data) {
+  return guarded([&] {
+    if (!path || ndims <= 0 || ndims > 16 || !dims || !data)
+      throw std::runtime_error("gtt_write_real: illegal arguments");
+    std::vector<size_t> d(dims, dims + ndims);
+    hoCuNDArray<float> a(d, const_cast<float *>(data));
+    write_nd_array(&a, path);
+  });
+}
+
+int gtt_read_real_header(const char *path, int *ndims, size_t *dims) {
+  return guarded([&] {
+    if (!path || !ndims || !dims) throw std::runtime_error("gtt_read_real_header: illegal arguments");
+    std::vector<size_t> d = read_nd_array_dims(path);
+    *ndims = (int)d.size();
+    for (size_t i = 0; i < d.size(); ++i) dims[i] = d[i];
+  });
+}
+
+int gtt_read_real(const char *path, float *data, size_t n) {
+  return guarded([&] {
+    if (!path || !data) throw std::runtime_error("gtt_read_real: illegal arguments");
+    hoCuNDArray<float> a = read_nd_array(path);
+    if (a.get_number_of_elements() != n)
+      throw std::runtime_error("gtt_read_real: the file holds " + std::to_string(a.get_number_of_elements()) +
+                               " elements, not " + std::to_string(n));
+    std::memcpy(data, a.get_data_ptr(), a.get_number_of_bytes());
   });
 }
 

@@ -31,9 +31,8 @@ inline void write_nd_array(const cuNDArray<float> *a, const std::string &path) {
   write_nd_array(&h, path);
 }
 
-inline hoCuNDArray<float> read_nd_array(const std::string &path) {
-  std::ifstream f(path, std::ios::binary);
-  if (!f) throw std::runtime_error("read_nd_array: cannot open " + path);
+// The sizes in a file's header (first index fastest); `f` is left at the data.
+inline std::vector<size_t> read_nd_array_header(std::ifstream &f, const std::string &path) {
   uint32_t nd = 0;
   f.read(reinterpret_cast<char *>(&nd), 4);
   if (!f || nd == 0 || nd > 16) throw std::runtime_error("read_nd_array: bad header in " + path);
@@ -43,6 +42,20 @@ inline hoCuNDArray<float> read_nd_array(const std::string &path) {
     f.read(reinterpret_cast<char *>(&v), 4);
     d = v;
   }
+  if (!f) throw std::runtime_error("read_nd_array: bad header in " + path);
+  return dims;
+}
+
+inline std::vector<size_t> read_nd_array_dims(const std::string &path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("read_nd_array: cannot open " + path);
+  return read_nd_array_header(f, path);
+}
+
+inline hoCuNDArray<float> read_nd_array(const std::string &path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("read_nd_array: cannot open " + path);
+  const std::vector<size_t> dims = read_nd_array_header(f, path);
   hoCuNDArray<float> a(dims);
   f.read(reinterpret_cast<char *>(a.get_data_ptr()), a.get_number_of_bytes());
   if (!f) throw std::runtime_error("read_nd_array: truncated data in " + path);

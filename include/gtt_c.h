@@ -114,6 +114,72 @@ int gtt_subset_op_mult_M(gtt_subset_op op, float *vol_dev, float *proj_dev, int 
 int gtt_subset_op_mult_MH(gtt_subset_op op, float *proj_dev, float *vol_dev, int subset, int accumulate);
 
 /* ------------------------------------------------------------------------
+ * cuConebeamProjectionOperator (xray/cuConebeamProjectionOperator.{h,cpp}):
+ * the device-resident projector over the bins of an acquisition.  Domain
+ * (nx, ny, nz[, nbins]) device volumes; codomain the bins' views in bin
+ * order (nu, nv, sum of bin sizes), device.  use_offset_correction: 0 / 1
+ * forces it (set_use_offset_correction), -1 lets setup decide (mean u
+ * offset above a tenth of the detector width, .h / .cpp:183-189).
+ */
+typedef struct gtt_cu_op_s *gtt_cu_op;
+int gtt_cu_op_create(gtt_cu_op *out);
+int gtt_cu_op_destroy(gtt_cu_op op);
+/* setup(acquisition, binning, is_dims_in_mm, transform_angles) (:148-215) */
+int gtt_cu_op_setup(gtt_cu_op op, const cbct_geom *g, const float *angles_deg, const float *offsets,
+                    int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
+                    int transform_angles, int use_offset_correction);
+/* mult_M (:38-80) and mult_MH (:82-116, full half-fan weights on a copy of
+ * the projections when offset correction is on, then the mask if set). */
+int gtt_cu_op_mult_M(gtt_cu_op op, const float *vol_dev, float *proj_dev, int accumulate);
+int gtt_cu_op_mult_MH(gtt_cu_op op, const float *proj_dev, float *vol_dev, int accumulate);
+/* calculate_mask (:118-146): per bin, the space-carving mask of the bin's
+ * projections (conebeam_spacecarver, xray/conebeam_projection.cu:822-921,
+ * 1043-1167) into mask_dev (the domain's size, one byte per voxel). */
+int gtt_cu_op_calculate_mask(gtt_cu_op op, const float *proj_dev, float limit,
+                             unsigned char *mask_dev);
+/* set_mask: mult_MH multiplies its result by the mask (apply_mask,
+ * xray/conebeam_projection.cu:1651-1657).  The mask is copied; NULL clears. */
+int gtt_cu_op_set_mask(gtt_cu_op op, const unsigned char *mask_dev);
+/* The decisions setup made: short scan, offset correction. */
+int gtt_cu_op_flags(gtt_cu_op op, int *short_scan, int *use_offset_correction);
+
+/* ------------------------------------------------------------------------
+ * hoCuConebeamProjectionOperator (xray/hoCuConebeamProjectionOperator.{h,cpp}):
+ * host arrays in and out.  setup shifts the angles to start at zero
+ * (transform_angles), detects short scans and, with use_offset_correction
+ * = -1, turns offset correction on for a mean u offset above a tenth of the
+ * detector width (.h:39-93).
+ *   mult_M (.cpp:105-176): per bin, FP of the bin's volume into the bin's
+ *     views of proj_host (other views untouched; accumulate adds the
+ *     projection, taking the views of no bin as zero); then, unless in FBP
+ *     mode, the square-root half-fan weights on every view (.cpp:168-169).
+ *   mult_MH (.cpp:179-266): FBP mode: per bin, cosine weights, [Parker],
+ *     ramp filter (filter_host, nu+1 taps, or the default filter),
+ *     [full half-fan weights], FDK-weighted BP, then image *= 2/N_bin
+ *     (.cpp:253); otherwise [square-root half-fan weights] and plain BP.
+ */
+typedef struct gtt_ho_op_s *gtt_ho_op;
+int gtt_ho_op_create(gtt_ho_op *out);
+int gtt_ho_op_destroy(gtt_ho_op op);
+int gtt_ho_op_setup(gtt_ho_op op, const cbct_geom *g, const float *angles_deg, const float *offsets,
+                    int nviews, const unsigned *bin_index, const int *bin_sizes, int nbins,
+                    int use_fbp, int use_offset_correction, const float *filter_host,
+                    float samples_per_pixel);
+int gtt_ho_op_mult_M(gtt_ho_op op, const float *image_host, float *proj_host, int accumulate);
+int gtt_ho_op_mult_MH(gtt_ho_op op, const float *proj_host, float *image_host, int accumulate);
+int gtt_ho_op_flags(gtt_ho_op op, int *short_scan, int *use_offset_correction);
+
+/* ------------------------------------------------------------------------
+ * ".real" arrays (write_nd_array / python/readReal.py:41-76): uint32 ndims,
+ * ndims x uint32 sizes (first index fastest), float32 data.  Host memory.
+ */
+int gtt_write_real(const char *path, int ndims, const size_t *dims, const float *data);
+/* dims: room for 16 sizes. */
+int gtt_read_real_header(const char *path, int *ndims, size_t *dims);
+/* n: the number of elements the header announces. */
+int gtt_read_real(const char *path, float *data, size_t n);
+
+/* ------------------------------------------------------------------------
  * Reconstruction server: FDK and OS-SART requests of any size served by
  * worker lanes on one GPU.  No counterpart in the reference, which runs one
  * reconstruction per process (xray/FDK_reconstruct_3d.cpp:24-207,

@@ -329,6 +329,46 @@ void orc_bp_transpose(const orc_geom *g, const float *angles_deg, const float *o
       }
 }
 
+/* Space-carving mask, conebeam_spacecarver_kernel (conebeam_projection.cu:822-921):
+ * the voxel centre with its z coordinate negated (:854) is projected onto
+ * every view; the projection is sampled at the nearest texel (point filter,
+ * normalised coordinates, zero outside the detector: the mask texture,
+ * :41-42); views whose value is below `limit` and not zero are counted;
+ * mask = count < 3.  ambiguous[v] = 1 when, for some view, the voxel's
+ * detector coordinate lies within 1e-3 texel of a texel edge (FP32 rounding
+ * may pick the other texel there). */
+void orc_spacecarver(const orc_geom *g, const float *angles_deg, const float *offsets, int nviews,
+                     const double *proj, double limit, unsigned char *mask,
+                     unsigned char *ambiguous) {
+  const size_t npix = (size_t)g->nu * g->nv;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < g->nz; ++z)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        const double px = ((x + 0.5) / g->nx - 0.5) * g->lx;
+        const double py = ((y + 0.5) / g->ny - 0.5) * g->ly;
+        const double pz = -(((z + 0.5) / g->nz - 0.5) * g->lz);
+        int count = 0, amb = 0;
+        for (int k = 0; k < nviews; ++k) {
+          const double th = angles_deg[k] * (ORC_PI / 180.0);
+          const double c = cos(th), sn = sin(th);
+          const double qx = c * px + sn * py, qy = -sn * px + c * py;
+          const double t = g->sdd / (qy + g->sad);
+          const double eu = qx * t - offsets[2 * k], ev = pz * t - offsets[2 * k + 1];
+          const double tu = (eu / g->fu + 0.5) * g->nu, tv = (ev / g->fv + 0.5) * g->nv;
+          const double fu = floor(tu), fv = floor(tv);
+          if (fmin(fmin(tu - fu, fu + 1.0 - tu), fmin(tv - fv, fv + 1.0 - tv)) < 1e-3) amb = 1;
+          double val = 0.0;
+          if (fu >= 0 && fu < g->nu && fv >= 0 && fv < g->nv)
+            val = proj[(size_t)fu + (size_t)g->nu * (size_t)fv + npix * k];
+          if (val < limit && val != 0.0) ++count;
+        }
+        const size_t idx = x + (size_t)g->nx * (y + (size_t)g->ny * z);
+        mask[idx] = count < 3;
+        ambiguous[idx] = (unsigned char)amb;
+      }
+}
+
 /* Cosine weights C[u,v] = SAD / sqrt(SAD^2 + s^2 + t^2), sampled at u/Nu (not
  * pixel centres): xray/hoCuConebeamProjectionOperator.cpp:64-102. */
 void orc_cosine_weights(const orc_geom *g, double *w) {

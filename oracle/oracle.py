@@ -98,6 +98,8 @@ def _load():
         _lib.orc_fp_rays.argtypes = [g, f, f, d, ctypes.c_float, ctypes.c_size_t, P(i), d]
         _lib.orc_bp_voxels.argtypes = [g, f, f, i, d, i, ctypes.c_size_t, P(ctypes.c_longlong), d]
         _lib.orc_cosine_weights.argtypes = [g, d]
+        u8 = P(ctypes.c_ubyte)
+        _lib.orc_spacecarver.argtypes = [g, f, f, i, d, ctypes.c_double, u8, u8]
         _lib.orc_guo_filter.argtypes = [i, d]
         _lib.orc_filter_kernel.argtypes = [i, d, d]
         _lib.orc_filter_rows.argtypes = [i, ctypes.c_size_t, d, d]
@@ -242,6 +244,27 @@ def bp_transpose(geom: Geom, angles, offsets, vol, fbp=False, out=None, accumula
     lib.orc_bp_transpose(ctypes.byref(g), _fp_(a), _fp_(o), a.size, _dp(v), _dp(out),
                          int(fbp), int(accumulate))
     return out
+
+
+def spacecarver(geom: Geom, angles, offsets, proj, limit):
+    """Space-carving mask (xray/conebeam_projection.cu:822-921, 1043-1167):
+    returns (mask, ambiguous), uint8 volumes; ambiguous marks voxels whose
+    detector coordinate lies within 1e-3 texel of a texel edge in some view."""
+    lib = _load()
+    a, o = _views(angles, offsets)
+    p = np.ascontiguousarray(proj, dtype=np.float64).reshape(geom.proj_shape(a.size))
+    mask = np.zeros(geom.vol_shape, np.uint8)
+    amb = np.zeros(geom.vol_shape, np.uint8)
+    g = geom.c()
+    u8 = ctypes.POINTER(ctypes.c_ubyte)
+    lib.orc_spacecarver(ctypes.byref(g), _fp_(a), _fp_(o), a.size, _dp(p), float(limit),
+                        mask.ctypes.data_as(u8), amb.ctypes.data_as(u8))
+    return mask, amb
+
+
+def apply_mask(image, mask):
+    """image * mask (apply_mask, xray/conebeam_projection.cu:1651-1657)."""
+    return np.asarray(image, np.float64) * (np.asarray(mask) != 0)
 
 
 # --------------------------------------------------------------------------
@@ -706,6 +729,39 @@ def fdk_mult_MH(geom: Geom, angles, offsets, proj, binning: Binning, F, short_sc
         # volume becomes 0 * inf = NaN there, and here
         with np.errstate(invalid="ignore"):
             view *= np.float32(2.0) / np.float32(len(bin_)) if len(bin_) else np.inf
+        of[b] = view.reshape(-1)
+    return out
+
+
+def ho_mult_MH(geom: Geom, angles, offsets, proj, binning: Binning, use_fbp=False, F=None,
+               short_scan=None, use_offset_correction=None, out=None, accumulate=False):
+    """hoCuConebeamProjectionOperator::mult_MH
+    (xray/hoCuConebeamProjectionOperator.cpp:179-266): the FBP branch is
+    :func:`fdk_mult_MH`; otherwise per bin the square-root half-fan weights
+    when offset correction is on (conebeam_projection.cu:1522-1523) and the
+    plain backprojection.  ``angles`` must already be shifted by setup."""
+    angles = np.asarray(angles, np.float32)
+    offsets = np.asarray(offsets, np.float32).reshape(-1, 2)
+    if use_offset_correction is None:
+        use_offset_correction = _auto_offset(geom, offsets)
+    if use_fbp:
+        return fdk_mult_MH(geom, angles, offsets, proj, binning, F, short_scan,
+                           use_offset_correction, out=out, accumulate=accumulate)
+    nb = len(binning)
+    shape = ((nb,) if nb > 1 else ()) + geom.vol_shape
+    if out is None:
+        out = np.zeros(shape, np.float64)
+    of = out.reshape(nb, -1)
+    for b, bin_ in enumerate(binning.bins):
+        view = np.ascontiguousarray(of[b]).reshape(geom.vol_shape)
+        if len(bin_) == 0:
+            if not accumulate:
+                view[...] = 0.0
+        else:
+            pb = np.array(np.asarray(proj, np.float64)[bin_])
+            if use_offset_correction:
+                pb = offset_weights(geom, offsets[bin_], pb, sqrt_variant=True)
+            bp(geom, angles[bin_], offsets[bin_], pb, fbp=False, out=view, accumulate=accumulate)
         of[b] = view.reshape(-1)
     return out
 

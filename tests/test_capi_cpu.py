@@ -105,3 +105,29 @@ def test_request_footprint_and_server_without_gpu():
         pass
     with pytest.raises(RuntimeError, match="gtt_server_create"):
         S.Server(lanes=2)
+
+
+def test_real_files_round_trip(tmp_path):
+    """.real arrays (write_nd_array; python/readReal.py:41-76): uint32 ndims,
+    the sizes first-index-fastest, float32 data.  libgtt's writer is read by
+    a numpy restatement of the reference's reader, and libgtt's reader reads
+    a file laid out as the reference's saveReal writes it."""
+    from gtt import operators as ops
+    a = np.random.default_rng(3).random((4, 5, 6)).astype(np.float32)
+    path = str(tmp_path / "a.real")
+    ops.write_real(path, a)
+    raw = np.fromfile(path, dtype=np.uint8)
+    hdr = raw[:16].view(np.uint32)
+    assert list(hdr) == [3, 6, 5, 4]                      # ndims, nx, ny, nz
+    assert np.array_equal(raw[16:].view(np.float32).reshape(4, 5, 6), a)
+    b = np.random.default_rng(4).random((2, 3, 7, 5)).astype(np.float32)
+    p2 = str(tmp_path / "b.real")
+    with open(p2, "wb") as f:                              # saveReal's layout
+        np.array(b.ndim, np.uint32).tofile(f)
+        np.array(b.shape[::-1], np.uint32).tofile(f)
+        b.tofile(f)
+    assert np.array_equal(ops.read_real(p2), b)
+    with open(p2, "r+b") as f:                             # truncated data
+        f.truncate(4 + 4 * b.ndim + 4 * (b.size - 1))
+    with pytest.raises(RuntimeError, match="truncated"):
+        ops.read_real(p2)

@@ -254,13 +254,16 @@ def test_os_mom_matches_oracle(tv):
 
 @pytest.mark.parametrize("kind", ["ramlak", "default", "shortscan", "offset"])
 def test_fdk_reconstruct_matches_oracle(kind):
-    arc = 220.0 if kind == "shortscan" else 360.0
+    # 180 degrees over 40 views spans 175.5: below pi + 3 half fan angles, a short scan
+    arc = 180.0 if kind == "shortscan" else 360.0
     g, angles, offs, vol, p = problem(nx=24, nu=48, nv=32, nviews=40, arc=arc)
     if kind == "offset":
         offs = np.zeros((40, 2), np.float32)
         offs[:, 0] = 0.2 * g.fov_u   # > 0.1 FOV: offset correction switches on
         p = O.fp(to_oracle(g), angles, offs, vol).astype(np.float32)
     print("CASE fdk", kind, flush=True)
+    a0 = (angles - np.min(angles)).astype(np.float32)
+    assert O._short_scan(to_oracle(g), a0) == (kind == "shortscan")
     F = None if kind == "default" else O.ramlak_filter(g.nu).astype(np.float32)
     f, scale = G.fdk_reconstruct_3d(g, angles, offs, p, F)
     Fo = O.guo_filter(g.nu).astype(np.float32) if F is None else F
@@ -293,7 +296,7 @@ def test_fdk_reconstruct_4d_matches_oracle(kind):
     device-resident driver against the oracle's restatement in the
     reference's steps, with uneven interleaved bins, views in no bin and an
     empty bin (whose volume is s f here; the reference divides by zero)."""
-    arc = 220.0 if kind != "full" else 360.0
+    arc = 180.0 if kind != "full" else 360.0   # a short scan (Parker weights) when not full
     nviews = 45
     g, angles, offs, vol, p = problem(nx=20, nu=40, nv=28, nviews=nviews, arc=arc)
     if kind != "full":

@@ -915,3 +915,91 @@ def test_weighted_solvers_uniform_weights_cancel():
     g, angles, vol, b, op, pb, ow, pbw = weighted_pair(weights=wfun)
     assert np.allclose(O.os_sart(ow, pbw, 2, beta=0.8), O.os_sart(op, pb, 2, beta=0.8), rtol=1e-9, atol=1e-12)
     assert np.allclose(O.os_mom(ow, pbw, 2), O.os_mom(op, pb, 2), rtol=1e-9, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Space carving, masks and the host operator's backprojection
+# ---------------------------------------------------------------------------
+
+def np_spacecarver(g: O.Geom, angles, offsets, proj, limit):
+    """Independent restatement of conebeam_spacecarver_kernel
+    (xray/conebeam_projection.cu:822-921): z negated, nearest texel of the
+    normalised coordinate, zero outside, count of views with 0 != value <
+    limit, mask = count < 3."""
+    gc = g.c()
+    z, y, x = np.meshgrid(np.arange(g.nz), np.arange(g.ny), np.arange(g.nx), indexing="ij")
+    px = ((x + 0.5) / g.nx - 0.5) * gc.lx
+    py = ((y + 0.5) / g.ny - 0.5) * gc.ly
+    pz = -((z + 0.5) / g.nz - 0.5) * gc.lz
+    count = np.zeros(px.shape, int)
+    for k, ang in enumerate(angles):
+        th = math.radians(float(np.float32(ang)))
+        qx = math.cos(th) * px + math.sin(th) * py
+        qy = -math.sin(th) * px + math.cos(th) * py
+        t = gc.sdd / (qy + gc.sad)
+        ou, ov = (float(np.float32(offsets[k][0])), float(np.float32(offsets[k][1]))) \
+            if offsets is not None else (0.0, 0.0)
+        iu = np.floor(((qx * t - ou) / gc.fu + 0.5) * g.nu).astype(int)
+        iv = np.floor(((pz * t - ov) / gc.fv + 0.5) * g.nv).astype(int)
+        inside = (iu >= 0) & (iu < g.nu) & (iv >= 0) & (iv < g.nv)
+        val = np.where(inside, proj[k][np.clip(iv, 0, g.nv - 1), np.clip(iu, 0, g.nu - 1)], 0.0)
+        count += (val < limit) & (val != 0)
+    return (count < 3).astype(np.uint8)
+
+
+def test_spacecarver_matches_numpy_restatement():
+    g = small_geom(nx=11, ny=9, nz=10, nu=17, nv=15)
+    rng = np.random.default_rng(31)
+    angles = np.sort(rng.uniform(0, 360, 7)).astype(np.float32)
+    offs = rng.uniform(-1, 1, (7, 2)).astype(np.float32)
+    proj = rng.random((7, g.nv, g.nu))
+    proj[proj < 0.2] = 0.0
+    mask, amb = O.spacecarver(g, angles, offs, proj, 0.6)
+    ref = np_spacecarver(g, angles, offs, proj, 0.6)
+    ok = amb == 0
+    assert ok.mean() > 0.9
+    assert np.array_equal(mask[ok], ref[ok])
+    assert 0 < mask.sum() < mask.size
+
+
+def test_spacecarver_known_answers():
+    """A constant sinogram c on a detector that covers the volume: every view
+    counts when c < limit (mask 0), none when c >= limit or c = 0 (mask 1).
+    Rows of the lower detector half only: the z negation sends them to the
+    upper half of the volume."""
+    g = small_geom(nx=8, nu=40, nv=40)
+    angles = np.arange(6, dtype=np.float32) * np.float32(60)
+    ones = np.ones((6, g.nv, g.nu))
+    assert O.spacecarver(g, angles, None, 0.5 * ones, 1.0)[0].sum() == 0
+    assert O.spacecarver(g, angles, None, 0.5 * ones, 0.5)[0].min() == 1
+    assert O.spacecarver(g, angles, None, 0.0 * ones, 1.0)[0].min() == 1
+    low = 0.5 * ones
+    low[:, g.nv // 2:, :] = 0.0          # only rows below the centre are nonzero
+    m, amb = O.spacecarver(g, angles, None, low, 1.0)
+    assert np.all(m[g.nz // 2:] == 0) and np.all(m[:g.nz // 2] == 1)
+
+
+def test_apply_mask_multiplies():
+    img = np.array([1.5, -2.0, np.inf, 3.0])
+    m = np.array([1, 0, 1, 0], np.uint8)
+    assert np.array_equal(O.apply_mask(img, m), [1.5, -0.0, np.inf, 0.0])
+
+
+def test_ho_mult_MH_plain_bins_and_sqrt_offset_weights():
+    """Non-FBP host backprojection: per bin the plain BP of the bin's views
+    into the bin's volume, after the square-root half-fan weights when
+    offset correction is on (conebeam_projection.cu:1522-1523)."""
+    g = small_geom(nx=8, nu=14, nv=10)
+    angles = np.arange(6, dtype=np.float32) * np.float32(60)
+    offs = np.zeros((6, 2), np.float32)
+    offs[:, 0] = 0.3 * g.fu                     # > 0.1 FOV: correction on by default
+    proj = np.random.default_rng(3).random((6, g.nv, g.nu))
+    b = O.Binning([[0, 2, 4], [1, 3, 5]])
+    out = O.ho_mult_MH(g, angles, offs, proj, b)
+    w = O.offset_weights(g, offs, np.ones_like(proj), sqrt_variant=True)
+    assert not np.allclose(w, 1.0)
+    for k, idx in enumerate(b.bins):
+        ref = O.bp(g, angles[idx], offs[idx], proj[idx] * w[idx])
+        assert np.allclose(out[k], ref, rtol=1e-13, atol=1e-15)
+    plain = O.ho_mult_MH(g, angles, offs, proj, b, use_offset_correction=False)
+    assert np.allclose(plain[1], O.bp(g, angles[[1, 3, 5]], offs[[1, 3, 5]], proj[[1, 3, 5]]))
