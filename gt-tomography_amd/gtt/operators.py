@@ -260,6 +260,7 @@ def _bind_extra(L):
         "gtt_ho_op_mult_M": [P, P, P, I], "gtt_ho_op_mult_MH": [P, P, P, I],
         "gtt_ho_op_flags": [P, IP, IP],
         "gtt_write_real": [ctypes.c_char_p, I, ctypes.POINTER(ctypes.c_size_t), P],
+        "gtt_cbct_forwards_projection": [G, F, F, I, P, P, P, I, F, P, P, P],
         "gtt_read_real_header": [ctypes.c_char_p, IP, ctypes.POINTER(ctypes.c_size_t)],
         "gtt_read_real": [ctypes.c_char_p, P, ctypes.c_size_t],
     }.items():
@@ -381,3 +382,22 @@ def read_real(path):
     out = np.empty(shape, np.float32)
     _check(L.gtt_read_real(os.fsencode(path), _p(out), out.size))
     return out
+
+
+def cbct_forwards_projection(g: capi.Geom, image, start_angle, angular_spacing, nviews,
+                             offsets=None, bins=None, samples_per_pixel=1.5):
+    """CBCT_forwards_projection's computation: returns (projections
+    (nviews, nv, nu), angles) for a host image (nz, ny, nx) or
+    (nbins, nz, ny, nx)."""
+    L = _bind_extra(load())
+    _, o = _views(np.zeros(nviews, np.float32), offsets)
+    idx, sizes, nb = _bins(bins)
+    x = np.ascontiguousarray(image, np.float32)
+    if x.shape != _vol_shape(g, max(nb, 1)):
+        raise ValueError("image must be %s" % (_vol_shape(g, max(nb, 1)),))
+    proj = np.empty((nviews, g.nv, g.nu), np.float32)
+    angles = np.empty(nviews, np.float32)
+    _check(L.gtt_cbct_forwards_projection(ctypes.byref(g), float(start_angle), float(angular_spacing),
+                                          int(nviews), _p(o), _p(idx), _p(sizes), nb,
+                                          float(samples_per_pixel), _p(x), _p(proj), _p(angles)))
+    return proj, angles

@@ -490,4 +490,30 @@ int gtt_read_real(const char *path, float *data, size_t n) {
   });
 }
 
+int gtt_cbct_forwards_projection(const cbct_geom *g, float start_angle, float angular_spacing,
+                                 int nviews, const float *offsets, const unsigned *bin_index,
+                                 const int *bin_sizes, int nbins, float spp, const float *image,
+                                 float *proj_out, float *angles_out) {
+  return guarded([&] {
+    if (!g || nviews <= 0 || !image || !proj_out)
+      throw std::runtime_error("CBCT_forwards_projection: illegal arguments");
+    std::vector<float> angles(nviews);
+    for (int i = 0; i < nviews; ++i) angles[i] = start_angle + (float)(unsigned)i * angular_spacing;
+    if (angles_out) std::memcpy(angles_out, angles.data(), sizeof(float) * nviews);
+    Scan s = make_scan(g, angles.data(), offsets, nviews, bin_index, bin_sizes, nbins, nullptr);
+    if (s.binning->get_maximum_projection_index() >= nviews)
+      throw std::runtime_error("Maximum projection index in binning file exceeds the number of projections requested");
+    std::vector<size_t> ps_dims{(size_t)g->nu, (size_t)g->nv, (size_t)nviews};
+    hoCuNDArray<float> projections(ps_dims, proj_out);
+    std::memset(proj_out, 0, projections.get_number_of_bytes());   // the binning may not write every view
+    hoCuConebeamProjectionOperator E;
+    if (spp > 0) E.set_num_samples_per_pixel(spp);
+    E.setup(s.acq, s.binning, s.dims_mm);
+    std::vector<size_t> idims = s.vol_dims;
+    if (idims.size() == 3) idims.push_back(1);
+    hoCuNDArray<float> img(idims, const_cast<float *>(image));
+    E.mult_M(&img, &projections);
+  });
+}
+
 }  // extern "C"

@@ -169,6 +169,21 @@ int gtt_ho_op_mult_M(gtt_ho_op op, const float *image_host, float *proj_host, in
 int gtt_ho_op_mult_MH(gtt_ho_op op, const float *proj_host, float *image_host, int accumulate);
 int gtt_ho_op_flags(gtt_ho_op op, int *short_scan, int *use_offset_correction);
 
+/* The computation of CBCT_forwards_projection
+ * (xray/CBCT_forwards_projection.cpp:60-254): angles start + i*spacing
+ * (float arithmetic, :167-172), zero-filled projections (:197), a
+ * hoCuConebeamProjectionOperator set up on the acquisition and binning
+ * (nbins <= 0: one bin of all views) with the automatic offset correction
+ * (so half-fan offsets put square-root weights on the result), then mult_M
+ * of the image (nx, ny, nz[, nbins], host) into projections_out (nu, nv,
+ * nviews, host).  angles_out (nullable) receives the angles as generated.
+ * The program's file I/O is gtt_write_real / gtt_h5_acquisition_save. */
+int gtt_cbct_forwards_projection(const cbct_geom *g, float start_angle, float angular_spacing,
+                                 int nviews, const float *offsets, const unsigned *bin_index,
+                                 const int *bin_sizes, int nbins, float samples_per_pixel,
+                                 const float *image_host, float *projections_out,
+                                 float *angles_out);
+
 /* ------------------------------------------------------------------------
  * ".real" arrays (write_nd_array / python/readReal.py:41-76): uint32 ndims,
  * ndims x uint32 sizes (first index fastest), float32 data.  Host memory.
