@@ -294,6 +294,8 @@ int cbct_views_set(cbct_ctx ctx, const float *angles_deg, const float *offsets, 
   CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_views_f, vf.data(), sizeof(ViewF) * n, hipMemcpyHostToDevice));
   CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_angles, angles_deg, sizeof(float) * n, hipMemcpyHostToDevice));
   ctx->nviews = n;
+  // the event that marks the end of the calls reading this table
+  if (!ctx->done) CBCT_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming));
   return CBCT_OK;
 }
 

@@ -42,6 +42,13 @@
 
 #include "cbct_internal.h"
 
+// Working copy of cbct_fp_noalloc: device memory the library owns for the
+// life of the process (one copy per GPU, each at its own address), so that
+// call allocates nothing.  Host code uses the per-device address from
+// hipGetSymbolAddress, never this name.
+constexpr size_t FP_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[FP_SCRATCH_BYTES];
+
 namespace {
 
 struct FPParams {
@@ -1036,19 +1043,33 @@ FPChoice select_variant(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int
   return default_variant(ctx, g, P, view0, nviews, per_step);
 }
 
-// TI: the caller's volume type; T: the padded working copy's type.
+// Bytes of the padded working copy cbct_fp builds for this call (the
+// variant's layout, element size esz).
+size_t fp_working_bytes(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
+                        size_t esz, float spp) {
+  const FPChoice choice = select_variant(ctx, g, P, view0, nviews, esz, spp);
+  if (choice.variant == 0) return (size_t)P.pxy * padded_dim(g->nz) * esz;
+  return ((size_t)P.pzx * padded_dim(g->ny) + 1024) * esz;
+}
+
+// TI: the caller's volume type; T: the padded working copy's type.  The
+// working copy goes to the context's workspace, or to `ext` when given (the
+// caller has checked it holds fp_working_bytes).
 template <typename TI, typename T>
 int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
-           const TI *vol, float *proj, int accumulate, hipStream_t s, float spp) {
+           const TI *vol, float *proj, int accumulate, hipStream_t s, float spp,
+           void *ext = nullptr) {
   const size_t esz = sizeof(T);
   const ViewD *views = ctx->d_views_d + view0;
   const size_t vt_bytes = (size_t)P.pzx * padded_dim(g->ny) * esz;
   const FPChoice choice = select_variant(ctx, g, P, view0, nviews, esz, spp);
   if (choice.variant == 0) {
     const size_t npad = (size_t)P.pxy * padded_dim(g->nz);
-    int rc = cbct_reserve(ctx, ctx->ws->vol_pad, npad * esz);
-    if (rc) return rc;
-    T *vp = static_cast<T *>(ctx->ws->vol_pad.ptr);
+    if (!ext) {
+      int rc = cbct_reserve(ctx, ctx->ws->vol_pad, npad * esz);
+      if (rc) return rc;
+    }
+    T *vp = static_cast<T *>(ext ? ext : ctx->ws->vol_pad.ptr);
     hipLaunchKernelGGL((k_pad_volume<TI, T>), stride_grid(npad), dim3(256), 0, s, vol, vp, g->nx, g->ny,
                        g->nz, P.px, (unsigned)padded_dim(g->ny), (unsigned)padded_dim(g->nz));
     CBCT_LAUNCH_CHECK(ctx, "k_pad_volume");
@@ -1064,9 +1085,11 @@ int run_fp(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int n
   const size_t npad = (size_t)P.pzx * py;
   // z-line chunk loads may run up to 127 elements past a column's end (those
   // values are never used); the slack keeps such reads inside this allocation
-  int rc = cbct_reserve(ctx, ctx->ws->vol_pad, (npad + 1024) * esz);
-  if (rc) return rc;
-  T *vt = static_cast<T *>(ctx->ws->vol_pad.ptr);
+  if (!ext) {
+    int rc = cbct_reserve(ctx, ctx->ws->vol_pad, (npad + 1024) * esz);
+    if (rc) return rc;
+  }
+  T *vt = static_cast<T *>(ext ? ext : ctx->ws->vol_pad.ptr);
   dim3 tgrid(ceil_div(P.pz, 32), ceil_div(P.px, 32), (unsigned)py);
   hipLaunchKernelGGL((k_pad_transpose<TI, T>), tgrid, dim3(32, 8), 0, s, vol, vt, g->nx, g->ny, g->nz,
                      P.px, P.pz);
@@ -1232,9 +1255,13 @@ const char *cbct_fp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nvie
   return v == 0 ? "k_fp_ray" : v >= 3 ? "k_fp_colc" : "k_fp_col";
 }
 
-int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol, int dtype,
-            float *proj, float spp, int accumulate, void *stream) {
-  if (!ctx) return CBCT_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Argument checks and parameters shared by cbct_fp and cbct_fp_noalloc.
+int fp_prepare(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol, int dtype,
+               float *proj, float spp, FPParams &P) {
   const char *who = "conebeam_forwards_projection";
   if (!vol || !proj)
     return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: illegal array pointer provided", who);
@@ -1244,27 +1271,90 @@ int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
   if (rc) return rc;
   if (dtype != CBCT_F32 && dtype != CBCT_BF16)
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "Error: %s: unsupported volume dtype %d", who, dtype);
-  if (nviews == 0) return CBCT_OK;
+  if (nviews == 0) return CBCT_OK;   // nothing to do (the caller returns)
   if (nviews > 65535)
     return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: at most 65535 views per call", who);
   if ((size_t)g->nu * (size_t)((g->nv + 63) / 64) * (size_t)nviews >= (1ull << 31))
     return cbct_fail(ctx, CBCT_ERR_ARG, "Error: %s: too many rays per call", who);
-  FPParams P;
   rc = make_params(ctx, g, spp, P);
   if (rc) return rc;
-  rc = cbct_set_device(ctx);
-  if (rc) return rc;
+  return cbct_set_device(ctx);
+}
+
+template <typename TI, typename T>
+int run_fp_as(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
+              const void *vol, float *proj, int accumulate, hipStream_t s, float spp, void *ext) {
+  return run_fp<TI, T>(ctx, g, P, view0, nviews, static_cast<const TI *>(vol), proj, accumulate, s,
+                       spp, ext);
+}
+
+int run_fp_dtype(cbct_ctx ctx, const cbct_geom *g, const FPParams &P, int view0, int nviews,
+                 const void *vol, int dtype, float *proj, int accumulate, hipStream_t s, float spp,
+                 void *ext) {
+  if (dtype == CBCT_BF16)
+    return run_fp_as<unsigned short, unsigned short>(ctx, g, P, view0, nviews, vol, proj, accumulate,
+                                                     s, spp, ext);
+  if (ctx->work_dtype == CBCT_BF16)
+    return run_fp_as<float, unsigned short>(ctx, g, P, view0, nviews, vol, proj, accumulate, s, spp,
+                                            ext);
+  return run_fp_as<float, float>(ctx, g, P, view0, nviews, vol, proj, accumulate, s, spp, ext);
+}
+
+// Per-device address of `scratch`, looked up on a device's first
+// cbct_fp_noalloc call.
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};
+
+int fp_scratch(cbct_ctx ctx, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (ctx->device < 0 || ctx->device >= 64)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_fp_noalloc: device %d out of range", ctx->device);
+  if (!g_scratch_ptr[ctx->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return cbct_fail(ctx, CBCT_ERR_HIP, "cbct_fp_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = p;
+  }
+  *out = g_scratch_ptr[ctx->device];
+  return CBCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol, int dtype,
+            float *proj, float spp, int accumulate, void *stream) {
+  if (!ctx) return CBCT_ERR_ARG;
+  FPParams P;
+  int rc = fp_prepare(ctx, g, view0, nviews, vol, dtype, proj, spp, P);
+  if (rc || nviews == 0) return rc;
   hipStream_t s = as_stream(stream);
   CallScope scope(ctx, s);
-  if (dtype == CBCT_BF16)
-    return run_fp<unsigned short, unsigned short>(ctx, g, P, view0, nviews,
-                                                  static_cast<const unsigned short *>(vol), proj,
-                                                  accumulate, s, spp);
-  if (ctx->work_dtype == CBCT_BF16)
-    return run_fp<float, unsigned short>(ctx, g, P, view0, nviews, static_cast<const float *>(vol),
-                                         proj, accumulate, s, spp);
-  return run_fp<float, float>(ctx, g, P, view0, nviews, static_cast<const float *>(vol), proj,
-                              accumulate, s, spp);
+  return run_fp_dtype(ctx, g, P, view0, nviews, vol, dtype, proj, accumulate, s, spp, nullptr);
+}
+
+int cbct_fp_noalloc(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol,
+                    int dtype, float *proj, float spp, int accumulate, void *stream) {
+  if (!ctx) return CBCT_ERR_ARG;
+  FPParams P;
+  int rc = fp_prepare(ctx, g, view0, nviews, vol, dtype, proj, spp, P);
+  if (rc || nviews == 0) return rc;
+  const size_t esz = (dtype == CBCT_BF16 || ctx->work_dtype == CBCT_BF16) ? 2 : 4;
+  const size_t need = fp_working_bytes(ctx, g, P, view0, nviews, esz, spp);
+  if (need > FP_SCRATCH_BYTES)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED,
+                     "cbct_fp_noalloc: the volume's working copy needs %zu bytes, more than the %zu "
+                     "bytes of scratch (use cbct_fp)", need, FP_SCRATCH_BYTES);
+  void *ptr = nullptr;
+  rc = fp_scratch(ctx, &ptr);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  rc = run_fp_dtype(ctx, g, P, view0, nviews, vol, dtype, proj, accumulate, s, spp, ptr);
+  // this call read the view table: a later cbct_views_set waits for it
+  if (rc == CBCT_OK && ctx->done && hipEventRecord(ctx->done, s) == hipSuccess) ctx->done_recorded = true;
+  return rc;
 }
 
 int cbct_fp_transpose(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const float *proj,

@@ -101,6 +101,7 @@ def load():
         "cbct_views_set": (I, [P, P, P, I]),
         "cbct_views_count": (I, [P]),
         "cbct_fp": (I, [P, G, I, I, P, I, P, F, I, P]),
+        "cbct_fp_noalloc": (I, [P, G, I, I, P, I, P, F, I, P]),
         "cbct_bp": (I, [P, G, I, I, P, I, P, I, I, P]),
         "cbct_fp_transpose": (I, [P, G, I, I, P, P, F, I, P]),
         "cbct_fp_engine": (I, [P, I]),
@@ -260,6 +261,16 @@ class Context:
         self._check(self.lib.cbct_fp(self.h, ctypes.byref(g), view0, nviews, _ptr(vol),
                                      _dtype_code(vol), _ptr(proj), samples_per_pixel,
                                      int(accumulate), _stream(stream)))
+        return proj
+
+    def fp_noalloc(self, g: Geom, vol, proj, view0=0, nviews=None, samples_per_pixel=1.5,
+                   accumulate=False, stream=None):
+        """cbct_fp_noalloc: fp() with the working copy in the library's fixed
+        per-GPU scratch (64 MiB); allocates nothing."""
+        nviews = proj.shape[0] if nviews is None else nviews
+        self._check(self.lib.cbct_fp_noalloc(self.h, ctypes.byref(g), view0, nviews, _ptr(vol),
+                                             _dtype_code(vol), _ptr(proj), samples_per_pixel,
+                                             int(accumulate), _stream(stream)))
         return proj
 
     def fp_kernel(self, g: Geom, view0=0, nviews=None, dtype="f32", samples_per_pixel=1.5):

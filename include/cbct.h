@@ -105,6 +105,22 @@ int cbct_views_count(cbct_ctx ctx);
 int cbct_fp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol,
             int vol_dtype, float *proj, float samples_per_pixel, int accumulate, void *stream);
 
+/* cbct_fp that allocates nothing: the same kernels and the same bits, with
+ * the volume's padded working copy in library-owned device memory of 64 MiB
+ * per GPU (held until the process ends) instead of the context's workspace.
+ * Volumes whose working copy does not fit (about 250^3 FP32 or 320^3 BF16
+ * working copies) are refused with CBCT_ERR_UNSUPPORTED.  Every call on one
+ * GPU shares that memory: calls that overlap on one GPU (other streams,
+ * other threads, other contexts) race and must be ordered by the caller.
+ * The device's first call looks the memory's address up; later calls make
+ * no allocation of any kind.  No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc).  cbct_views_set must have been called
+ * on the context (it allocates the view table). */
+int cbct_fp_noalloc(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void *vol,
+                    int vol_dtype, float *proj, float samples_per_pixel, int accumulate,
+                    void *stream);
+
 /* Voxel-driven backprojection with bilinear detector interpolation.
  * vol = (accumulate ? vol : 0) + sum_views w * P(project(voxel)), w = 1 or the
  * FDK weight 1/U^2 when fbp != 0.  proj_dtype CBCT_F32 or CBCT_BF16.
