@@ -458,10 +458,10 @@ def serving_leg(ctx, lanes=4, n_batch=64, n_stream=32, load=0.8):
 
     from gtt import serving as S
 
-    def run(reqs, lanes_, arrivals=None):
+    def run(reqs, lanes_, arrivals=None, graphs=False):
         for r in reqs:
             r.volume = None
-        with S.Server(lanes=lanes_) as srv:
+        with S.Server(lanes=lanes_, graphs=graphs) as srv:
             res, wall = S.run_stream(srv, reqs, arrivals)
         return res, wall
 
@@ -475,6 +475,8 @@ def serving_leg(ctx, lanes=4, n_batch=64, n_stream=32, load=0.8):
     rp, wp = run(reqs, 1)
     vp = [dict(volume=r["volume"].copy()) for r in rp]
     rs, ws = run(reqs, lanes)
+    # served, iterations 2.. replayed as captured HIP graphs
+    rg, wg = run(reqs, lanes, graphs=True)
     # served with BF16 working copies, against the plain FP32 results
     for r in reqs:
         r.storage = "bf16"
@@ -486,13 +488,15 @@ def serving_leg(ctx, lanes=4, n_batch=64, n_stream=32, load=0.8):
     sb = S.summarize(reqs, rb, wb)
     sb["rel_l2_vs_plain_fp32"] = {"max": max(errs), "mean": float(np.mean(errs))}
     sp = S.summarize(reqs, rp, wp)
-    out["batch"] = {"plain": sp, "served": S.summarize(reqs, rs, ws), "served_bf16": sb,
+    out["batch"] = {"plain": sp, "served": S.summarize(reqs, rs, ws),
+                    "served_graphs": S.summarize(reqs, rg, wg),
+                    "graphs_identical": same(vp, rg), "served_bf16": sb,
                     "served_bf16_gups_vs_plain_fp32": round(sb["gups"] / sp["gups"], 3),
                     "identical": same(vp, rs),
                     "sizes": "%d OS-SART scans, edge %d..%d, %d..%d views" % (
                         len(reqs), min(r.geom.nx for r in reqs), max(r.geom.nx for r in reqs),
                         min(len(r.angles) for r in reqs), max(len(r.angles) for r in reqs))}
-    del reqs, rp, rs, rb, vp
+    del reqs, rp, rs, rb, rg, vp
     t0 = time.perf_counter()
     reqs = [S.make_request(s, ctx) for s in S.uneven_scans(n_stream, seed=1004, p_fdk=0.6)]
     gen_s += time.perf_counter() - t0
@@ -508,6 +512,65 @@ def serving_leg(ctx, lanes=4, n_batch=64, n_stream=32, load=0.8):
                      "plain": S.summarize(reqs, rp, wp), "served": S.summarize(reqs, rs, ws),
                      "identical": same(vp, rs)}
     out["request_generation_s"] = round(gen_s, 2)
+    return out
+
+
+def batch_trace(ctx, lanes, graphs, storage="f32", n_batch=64):
+    """One run of the uneven 64-scan OS-SART batch (for a kernel trace): the
+    clock readings around it bound the window trace_busy looks at."""
+    from gtt import serving as S
+    reqs = [S.make_request(s, ctx) for s in S.uneven_scans(n_batch, seed=1003)]
+    for r in reqs:
+        r.storage = storage
+    import torch
+    torch.cuda.synchronize()
+    time.sleep(0.5)
+    t0 = (time.monotonic_ns(), time.clock_gettime_ns(time.CLOCK_BOOTTIME))
+    with S.Server(lanes=lanes, graphs=graphs) as srv:
+        res, wall = S.run_stream(srv, reqs)
+    t1 = (time.monotonic_ns(), time.clock_gettime_ns(time.CLOCK_BOOTTIME))
+    out = {"lanes": lanes, "graphs": graphs, "storage": storage, "summary": S.summarize(reqs, res, wall),
+           "window_monotonic_ns": [t0[0], t1[0]], "window_boottime_ns": [t0[1], t1[1]]}
+    print(json.dumps(out), flush=True)
+    return out
+
+
+def trace_busy(dirs, window=None):
+    """Device occupancy of a kernel trace (rocprofv3 --kernel-trace CSV):
+    the union of all kernels' [start, end) intervals against the span from
+    the first start to the last end, the number of dispatches and the gaps
+    between busy intervals."""
+    import csv
+    import glob
+    iv = []
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    s, e = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+                    if window is None or (s >= window[0] and e <= window[1]):
+                        iv.append((s, e))
+    if not iv:
+        print(json.dumps({"dispatches": 0}))
+        return None
+    iv.sort()
+    busy, gaps = 0, []
+    cs, ce = iv[0]
+    for s, e in iv[1:]:
+        if s > ce:
+            busy += ce - cs
+            gaps.append(s - ce)
+            cs, ce = s, e
+        else:
+            ce = max(ce, e)
+    busy += ce - cs
+    span = iv[-1][1] - iv[0][0]
+    gaps.sort()
+    out = {"dispatches": len(iv), "span_ms": round(span / 1e6, 3), "busy_ms": round(busy / 1e6, 3),
+           "busy_frac": round(busy / span, 4), "idle_gaps": len(gaps),
+           "gap_p50_us": round(gaps[len(gaps) // 2] / 1e3, 2) if gaps else 0.0,
+           "gap_total_ms": round(sum(gaps) / 1e6, 3)}
+    print(json.dumps(out))
     return out
 
 
@@ -530,6 +593,14 @@ def main():
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="with --profile-steps: first run a plain 1 GiB device copy whose "
                          "FETCH_SIZE/WRITE_SIZE scale the projector kernels' counters")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the served-vs-plain batch and stream legs (for tracing)")
+    ap.add_argument("--trace-busy", nargs="+", metavar="DIR",
+                    help="device occupancy of rocprofv3 --kernel-trace CSV output (no GPU work)")
+    ap.add_argument("--window", nargs=2, type=int, metavar=("T0", "T1"),
+                    help="with --trace-busy: only dispatches inside [T0, T1] (ns, the trace's clock)")
+    ap.add_argument("--batch-trace", nargs="+", metavar="ARG",
+                    help="LANES [graphs] [bf16]: run the 64-scan batch once (for rocprofv3 --kernel-trace)")
     ap.add_argument("--pmc-summary", nargs="+", metavar="ARG",
                     help="KERNEL WORKLOAD {fp,bp,filter} DIR...: collect rocprofv3 --pmc CSV "
                          "output into profiles/pmc_KERNEL_WORKLOAD.json (no GPU work)")
@@ -540,6 +611,10 @@ def main():
         src = {"fp": FP_SOURCES, "bp": BP_SOURCES, "filter": FILTER_SOURCES}[which]
         summarize_pmc(kernel, workload, src, args.pmc_summary[3:],
                       os.path.join(ROOT, "profiles", "pmc_%s_%s.json" % (kernel, workload)))
+        return
+
+    if args.trace_busy:
+        trace_busy(args.trace_busy, args.window)
         return
 
     if args.gpus != 1:
@@ -562,6 +637,15 @@ def main():
     nsamp = int(np.float32(1.5) * np.float32(g.nx))
 
     ctx = capi.Context(0)
+    if args.serving_only:
+        print(json.dumps({"latency": serving_leg(ctx)}), flush=True)
+        ctx.close()
+        return
+    if args.batch_trace:
+        batch_trace(ctx, int(args.batch_trace[0]), "graphs" in args.batch_trace,
+                    "bf16" if "bf16" in args.batch_trace else "f32")
+        ctx.close()
+        return
     ctx.set_views(angles, cfg.offsets())
     vol = PH.voxelize(cfg.phantom(), cfg.vol_shape, cfg.vol_mm, device="cuda")
     torch.cuda.synchronize()

@@ -99,6 +99,11 @@ struct CallScope {
 };
 // Wait (host) for the end of the last call that read the context's tables.
 int cbct_wait_tables(cbct_ctx ctx);
+// Host-to-device copy of a context's table, complete on return.  It goes
+// through a non-blocking stream of the library's own (one per device), not
+// the legacy default stream, so it neither waits for nor breaks the work or
+// graph captures of other threads' streams.
+int cbct_upload(cbct_ctx ctx, void *dst, const void *src, size_t bytes);
 
 enum { PROF_FP = 0, PROF_BP = 1, PROF_FILTER = 2 };
 // Record the start/stop event of a profiled kernel on `s` (no-op unless enabled).

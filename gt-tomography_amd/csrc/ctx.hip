@@ -114,6 +114,22 @@ CallScope::~CallScope() {
   if (ctx->done && hipEventRecord(ctx->done, s) == hipSuccess) ctx->done_recorded = true;
 }
 
+namespace {
+std::mutex g_upload_mu;
+hipStream_t g_upload_stream[64] = {};
+}  // namespace
+
+int cbct_upload(cbct_ctx ctx, void *dst, const void *src, size_t bytes) {
+  if (ctx->device < 0 || ctx->device >= 64)
+    return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "device %d out of range", ctx->device);
+  std::lock_guard<std::mutex> lk(g_upload_mu);
+  hipStream_t &s = g_upload_stream[ctx->device];
+  if (!s) CBCT_HIP_CHECK(ctx, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  CBCT_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  CBCT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  return CBCT_OK;
+}
+
 int cbct_wait_tables(cbct_ctx ctx) {
   if (ctx->done_recorded) CBCT_HIP_CHECK(ctx, hipEventSynchronize(ctx->done));
   return CBCT_OK;
@@ -290,9 +306,9 @@ int cbct_views_set(cbct_ctx ctx, const float *angles_deg, const float *offsets, 
   // (this context's calls only, on whatever stream they were made).
   rc = cbct_wait_tables(ctx);
   if (rc) return rc;
-  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_views_d, vd.data(), sizeof(ViewD) * n, hipMemcpyHostToDevice));
-  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_views_f, vf.data(), sizeof(ViewF) * n, hipMemcpyHostToDevice));
-  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_angles, angles_deg, sizeof(float) * n, hipMemcpyHostToDevice));
+  if ((rc = cbct_upload(ctx, ctx->d_views_d, vd.data(), sizeof(ViewD) * n))) return rc;
+  if ((rc = cbct_upload(ctx, ctx->d_views_f, vf.data(), sizeof(ViewF) * n))) return rc;
+  if ((rc = cbct_upload(ctx, ctx->d_angles, angles_deg, sizeof(float) * n))) return rc;
   ctx->nviews = n;
   // the event that marks the end of the calls reading this table
   if (!ctx->done) CBCT_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming));

@@ -356,8 +356,10 @@ int ensure_toeplitz_f16(cbct_ctx ctx) {
     }
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_thi, sizeof(_Float16) * hi.size()));
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_tlo, sizeof(_Float16) * lo.size()));
-  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_thi, hi.data(), sizeof(_Float16) * hi.size(), hipMemcpyHostToDevice));
-  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_tlo, lo.data(), sizeof(_Float16) * lo.size(), hipMemcpyHostToDevice));
+  int rc = cbct_upload(ctx, ctx->d_thi, hi.data(), sizeof(_Float16) * hi.size());
+  if (rc) return rc;
+  rc = cbct_upload(ctx, ctx->d_tlo, lo.data(), sizeof(_Float16) * lo.size());
+  if (rc) return rc;
   ctx->toe_ld = (int)kp;
   ctx->toe_inv_scale = (float)std::ldexp(1.0, ex - 14);
   return CBCT_OK;
@@ -426,7 +428,8 @@ int ensure_cos_weights(cbct_ctx ctx, const cbct_geom *g) {
   ctx->d_cosw = nullptr;
   ctx->cosw_valid = false;
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_cosw, sizeof(float) * w.size()));
-  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_cosw, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+  rc = cbct_upload(ctx, ctx->d_cosw, w.data(), sizeof(float) * w.size());
+  if (rc) return rc;
   ctx->cosw_geom = *g;
   ctx->cosw_valid = true;
   return CBCT_OK;
@@ -507,7 +510,8 @@ int cbct_filter_set(cbct_ctx ctx, int nu, const float *F) {
   ctx->filter_nu = 0;
   ctx->h64 = h;
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_h, sizeof(float) * hm.size()));
-  CBCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_h, hm.data(), sizeof(float) * hm.size(), hipMemcpyHostToDevice));
+  rc = cbct_upload(ctx, ctx->d_h, hm.data(), sizeof(float) * hm.size());
+  if (rc) return rc;
   ctx->filter_nu = nu;
   return CBCT_OK;
 }

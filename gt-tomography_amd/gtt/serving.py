@@ -59,6 +59,8 @@ def _lib():
         L.gtt_server_last_error.restype = ctypes.c_char_p
         L.gtt_server_set_autotune.argtypes = [P, I]
         L.gtt_server_set_autotune.restype = I
+        L.gtt_server_set_graphs.argtypes = [P, I]
+        L.gtt_server_set_graphs.restype = I
         L.gtt_request_footprint.argtypes = [ctypes.POINTER(_Request)]
         L.gtt_request_footprint.restype = ctypes.c_size_t
         for n in ("gtt_server_create", "gtt_server_destroy", "gtt_server_submit", "gtt_server_wait"):
@@ -116,7 +118,7 @@ class Request:
 class Server:
     """gtt_server: `lanes` worker lanes (own HIP stream each) on one device."""
 
-    def __init__(self, lanes=4, hbm_budget_bytes=0, device=0, autotune=False):
+    def __init__(self, lanes=4, hbm_budget_bytes=0, device=0, autotune=False, graphs=False):
         L = _lib()
         h = ctypes.c_void_p()
         if L.gtt_server_create(device, lanes, hbm_budget_bytes, ctypes.byref(h)) != 0:
@@ -124,6 +126,8 @@ class Server:
         self.h = h
         if autotune:
             L.gtt_server_set_autotune(h, 1)
+        if graphs:
+            L.gtt_server_set_graphs(h, 1)
         self.lanes = lanes
         self._live = {}
 

@@ -55,6 +55,7 @@ struct Job {
   int status = 0;
   double scale = 0.0;
   bool autotune = false;   // tune the forward projector for the scan's geometry first
+  bool graphs = false;     // OS-SART iterations 2.. replayed as captured HIP graphs
   std::string message;
 };
 
@@ -115,6 +116,7 @@ void run_request(Job &j) {
     solver.set_beta(r.beta);
     solver.set_gamma(r.gamma);
     solver.set_non_negativity_constraint(r.nonneg != 0);
+    solver.set_graphs(j.graphs);
     auto x = solver.solve(&b);
     copy_d2h(r.volume_out, x->get_data_ptr(), x->get_number_of_bytes());
   }
@@ -125,6 +127,7 @@ void run_request(Job &j) {
 struct gtt_server_s {
   int device = 0;
   bool autotune = false;
+  bool graphs = false;
   size_t budget = 0;
   size_t in_use = 0;
   size_t kept = 0;                 // memory idle lanes keep (sum over lanes)
@@ -260,6 +263,13 @@ const char *gtt_server_last_error(gtt_server s) { return s ? s->err.c_str() : "n
 
 size_t gtt_request_footprint(const gtt_request *req) { return req ? footprint_bytes(*req) : 0; }
 
+int gtt_server_set_graphs(gtt_server s, int on) {
+  if (!s) return -1;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->graphs = on != 0;
+  return 0;
+}
+
 int gtt_server_set_autotune(gtt_server s, int on) {
   if (!s) return -1;
   std::lock_guard<std::mutex> lk(s->mu);
@@ -289,6 +299,7 @@ int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id) {
   auto j = std::make_shared<Job>();
   j->req = *req;
   j->autotune = s->autotune;
+  j->graphs = s->graphs;
   j->footprint = footprint_bytes(*req);
   if (j->footprint > s->budget) {
     s->err = "gtt_server_submit: request needs " + std::to_string(j->footprint) +

@@ -17,7 +17,9 @@
  *    never throws; cbct_last_error() returns the message.  The C++ operator
  *    layer rethrows these as std::runtime_error with the reference's text.
  *  - A context is single-threaded; distinct contexts may be used concurrently.
- *    There is no global mutable state.
+ *    Library-wide state is limited to one upload stream per device (table
+ *    uploads, serialised), the autotuner's table and cbct_fp_noalloc's
+ *    scratch; none of it touches the legacy default stream.
  *  - Calls on one context may use different streams: a call that uses the
  *    context's scratch first makes its stream wait for the previous such call
  *    when that was enqueued on another stream, so the scratch is never

@@ -243,6 +243,10 @@ int gtt_server_destroy(gtt_server s);  /* drains the queue first */
  * their geometry first (cbct_fp_autotune; choices are kept per geometry for
  * the process, so repeated geometries pay once).  Default off. */
 int gtt_server_set_autotune(gtt_server s, int on);
+/* OS-SART requests submitted from now on replay their iterations 2.. as
+ * captured HIP graphs on their lane's stream (gtt_set_graphs; same bits).
+ * Default off. */
+int gtt_server_set_graphs(gtt_server s, int on);
 int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id);
 int gtt_server_wait(gtt_server s, long long id, gtt_result *res);
 const char *gtt_server_last_error(gtt_server s);

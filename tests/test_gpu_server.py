@@ -195,3 +195,34 @@ def test_footprint_covers_measured_peak(ctx, kind):
     print("kind", kind, "peak", peak, "estimate", est, "ratio %.2f" % (peak / est))
     assert np.all(np.isfinite(out["volume"]))
     assert 0 < peak <= est
+
+
+def test_served_with_graphs_equals_alone(small_requests):
+    """OS-SART requests served with their iterations replayed as captured HIP
+    graphs (gtt_server_set_graphs) give the alone results bit for bit."""
+    print_case("server graphs", [(r.kind, r.geom.nx) for r in small_requests])
+    ref = [alone(r) for r in small_requests]
+    for r in small_requests:
+        r.volume = None
+    with S.Server(lanes=3, graphs=True) as srv:
+        res, _ = S.run_stream(srv, small_requests)
+    for r, out, want in zip(small_requests, res, ref):
+        assert np.array_equal(out["volume"], want), (r.kind, r.geom.nx)
+
+
+def test_graph_captures_beside_other_lanes_setup(ctx):
+    """Lanes that capture OS-SART iterations while other lanes set up their
+    operators (view-table and filter uploads) and run FDK requests: the
+    uploads go through the library's own non-blocking stream, never the
+    legacy default stream, which a capture on another stream forbids.
+    Results equal the alone runs bit for bit."""
+    specs = S.uneven_scans(16, seed=77, p_fdk=0.4, scale=0.35)
+    reqs = [S.make_request(s, ctx, iterations=4) for s in specs]
+    print_case("graphs beside setups", [(r.kind, r.geom.nx, len(r.angles)) for r in reqs])
+    ref = [alone(r) for r in reqs]
+    for r in reqs:
+        r.volume = None
+    with S.Server(lanes=4, graphs=True) as srv:
+        res, _ = S.run_stream(srv, reqs)
+    for r, out, want in zip(reqs, res, ref):
+        assert np.array_equal(out["volume"], want), (r.kind, r.geom.nx)
