@@ -706,7 +706,7 @@ def main():
 
     s_store = 4 if args.storage == "f32" else 2
     fp_kernel = ctx.fp_kernel(g, 0, nviews, args.storage)
-    bp_kernel = "k_bps" if args.storage == "f32" else "k_bp"
+    bp_kernel = ctx.bp_kernel(g, 0, nviews)
     extra = {"fp_ms": round(fp_ms, 3), "bp_ms": round(bp_ms, 3),
              "gups_fp": round(nvox * nviews / (fp_ms / 1e3) / 1e9, 2),
              "gups_bp": round(nvox * nviews / (bp_ms / 1e3) / 1e9, 2),

@@ -34,6 +34,10 @@ struct BPParams {
   unsigned pnu;        // padded row length
   unsigned pnv;        // padded rows
   unsigned pview;      // padded elements per view
+  unsigned pnu2;       // pair copy: row length (pnu rounded up to 4)
+  unsigned pview2;     // pair copy: dwords per view
+  unsigned bpp_rowb;   // LDS row of the pair kernel in bytes (a parameter, so
+                       // the compiler keeps iv * row + col one 24-bit mad)
 };
 
 template <typename T>
@@ -166,52 +170,48 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
   }
 }
 
-// LDS-staged variant (FP32 working copies).  Per view, the block's detector
-// rectangle (from the projections of its four corner columns at its first
-// and last z, which bound every voxel of the block: u is a linear-fractional
-// function of (x, y) and v is linear in the magnification, so both are
-// extreme at the corners; one texel of margin covers rounding) is copied
-// from the plain zero-bordered copy into LDS by direct global-to-LDS loads,
-// one view ahead (the next view's rectangle streams in while this view is
-// computed), and every voxel takes its four taps from there.  Same
-// arithmetic, same bits as k_bp.  Rectangles wider than 128 texels or
-// taller than the buffer are read tap by tap.
-constexpr int BPS_CAP = 4096;   // floats per LDS buffer
+// LDS-staged backprojector.  Per view, each block's detector rectangle
+// (k_bpp_rects) is copied into LDS by 16-byte direct global-to-LDS loads, one
+// view ahead, and every voxel takes its taps from there.  Two layouts, both
+// 4-byte elements in rows of pnu2 (16-byte aligned): PAIRS (BF16 working
+// copies, k_pad_pair): element (a, b) holds the taps (a, b) and (a+1, b), so
+// the four taps of a voxel-view are two elements one LDS row apart, one
+// ds_read2_b32; plain FP32 (k_pad_plain): the four taps are two adjacent
+// pairs of elements one row apart, two ds_read2_b32.  LDS rows are BPP_RW =
+// 68 elements (adjacent rows 4 banks apart); a buffer holds BPP_CAP
+// elements, a multiple of one 64-lane copy (256), so whole copies never write
+// past it.  Views whose rectangle does not fit read the copy directly.  Same
+// arithmetic, same bits as k_bp.
+constexpr int BPP_RW = 68;
+constexpr int BPP_CAP = 13 * 256;   // 48 rows of 68 elements, rounded up to whole copies
+constexpr int BPP_CHUNKS = BPP_CAP / 256;
 
-struct BPSRect {
-  int u0, v0, W, H, rw;   // origin, size, LDS row stride (64 or 128 floats)
-  bool staged;
-};
-
-template <bool FBP, int KZ>
-__global__ void __launch_bounds__(BP_TX *BP_TY) k_bps(BPParams P, const ViewF *__restrict__ views,
-                                                      const float *__restrict__ ppad,
-                                                      float *__restrict__ vol, int accumulate,
-                                                      unsigned gx, unsigned gy) {
+// The rectangle of block L (x fastest, z slowest, as k_bpp) at view k, as
+// entry L * nviews + k: (u0 | v0 << 16, H), H = 0 when it does not fit.
+// Its bounds: the projections of the block's four corner columns at its
+// first and last z bound every voxel of the block (u is a linear-fractional
+// function of (x, y), v is linear in the magnification, so both are extreme
+// at the corners), with one texel of margin for rounding (columns up to the
+// right tap's); u0 is rounded down to 4 elements so the copies are 16-byte
+// aligned.
+template <int KZ>
+__global__ void __launch_bounds__(256) k_bpp_rects(BPParams P, const ViewF *__restrict__ views,
+                                                   uint2 *__restrict__ rects, unsigned gx, unsigned gy,
+                                                   unsigned nblocks) {
 #pragma clang fp contract(off)
-  __shared__ float buf[2 * BPS_CAP];   // indexed directly: LDS instructions, not flat ones
-  const unsigned L = blockIdx.x;
-  const int tid = threadIdx.x + BP_TX * threadIdx.y, wv = tid >> 6, ln = tid & 63;
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (size_t)nblocks * P.nviews) return;
+  const unsigned L = (unsigned)(t / P.nviews), k = (unsigned)(t % P.nviews);
   const int xb = (int)(L % gx) * BP_TX, yb = (int)((L / gx) % gy) * BP_TY;
-  const int x = xb + (wv & 1) * 8 + (ln & 7);
-  const int y = yb + (wv >> 1) * 8 + (ln >> 3);
   const int z0 = (int)(L / (gx * gy)) * KZ;
-  const bool active = x < P.nx && y < P.ny;   // the others stage and synchronise only
-  const float px = ((x + 0.5f) / P.nx - 0.5f) * P.lx;
-  const float py = ((y + 0.5f) / P.ny - 0.5f) * P.ly;
   const float dz = P.lz / P.nz;
   const float pz0 = ((z0 + 0.5f) / P.nz - 0.5f) * P.lz;
-  // every lane evaluates the corner column ln & 3 of the block
-  const int cxi = xb + ((ln & 1) ? BP_TX - 1 : 0), cyi = yb + ((ln & 2) ? BP_TY - 1 : 0);
-  const float cpx = ((cxi + 0.5f) / P.nx - 0.5f) * P.lx;
-  const float cpy = ((cyi + 0.5f) / P.ny - 0.5f) * P.ly;
-  const int view_bytes = (int)(P.pview * sizeof(float));
-  auto rsrc_of = [&](int k) {
-    return __builtin_amdgcn_make_buffer_rsrc((void *)(ppad + (size_t)k * P.pview), (short)0, view_bytes,
-                                             0x00020000);
-  };
-  auto rect_of = [&](int k) {
-    const ViewF vw = views[k];
+  const ViewF vw = views[k];
+  int ulo = 1 << 30, uhi = -(1 << 30), vlo = 1 << 30, vhi = -(1 << 30);
+  for (int c = 0; c < 4; ++c) {
+    const int cxi = xb + ((c & 1) ? BP_TX - 1 : 0), cyi = yb + ((c & 2) ? BP_TY - 1 : 0);
+    const float cpx = ((cxi + 0.5f) / P.nx - 0.5f) * P.lx;
+    const float cpy = ((cyi + 0.5f) / P.ny - 0.5f) * P.ly;
     const float cqx = fmaf(vw.c, cpx, vw.s * cpy);
     const float cqy = fmaf(vw.c, cpy, -vw.s * cpx);
     const float cmag = P.sdd * __builtin_amdgcn_rcpf(cqy + P.sad);
@@ -219,40 +219,79 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bps(BPParams P, const ViewF *_
     const float ctv0 = fmaf(fmaf(pz0, cmag, -vw.off_v), P.sv, P.cv);
     const float ca = __builtin_amdgcn_fmed3f(ctv0, 0.0f, P.clv);
     const float cb = __builtin_amdgcn_fmed3f(fmaf((float)(KZ - 1), dz * cmag * P.sv, ctv0), 0.0f, P.clv);
-    int ulo = (int)ctu, uhi = ulo, vlo = (int)fminf(ca, cb), vhi = (int)fmaxf(ca, cb);
+    ulo = min(ulo, (int)ctu);
+    uhi = max(uhi, (int)ctu);
+    vlo = min(vlo, (int)fminf(ca, cb));
+    vhi = max(vhi, (int)fmaxf(ca, cb));
+  }
+  const int u0 = max(ulo - 1, 0) & ~3, v0 = max(vlo - 1, 0);
+  const int W = min(uhi + 2, (int)P.pnu2 - 1) - u0 + 1;
+  const int H = min(vhi + 2, (int)P.pnv - 1) - v0 + 1;
+  const bool staged = W <= BPP_RW && H * BPP_RW <= BPP_CAP;
+  rects[t] = make_uint2((unsigned)u0 | ((unsigned)v0 << 16), staged ? (unsigned)H : 0u);
+}
+
+template <bool PAIRS, bool FBP, int KZ>
+__global__ void __launch_bounds__(BP_TX *BP_TY) k_bpp(BPParams P, const ViewF *__restrict__ views,
+                                                      const unsigned *__restrict__ copy,
+                                                      const uint2 *__restrict__ rects,
+                                                      float *__restrict__ vol, int accumulate,
+                                                      unsigned gx, unsigned gy) {
+#pragma clang fp contract(off)
+  __shared__ unsigned buf[2 * BPP_CAP];
+  // linear tile id, x fastest, z slowest (consecutive tiles land on
+  // different XCDs, so all XCDs work on one band of detector rows per view)
+  const unsigned L = blockIdx.x;
+  const int tid = threadIdx.x + BP_TX * threadIdx.y, ln = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar branches
+  // a wave's 64 columns are an 8 x 8 quarter of the 16 x 16 tile
+  const int x = (int)(L % gx) * BP_TX + (wv & 1) * 8 + (ln & 7);
+  const int y = (int)((L / gx) % gy) * BP_TY + (wv >> 1) * 8 + (ln >> 3);
+  const int z0 = (int)(L / (gx * gy)) * KZ;
+  const bool active = x < P.nx && y < P.ny;   // the others stage and synchronise only
+  const float px = ((x + 0.5f) / P.nx - 0.5f) * P.lx;
+  const float py = ((y + 0.5f) / P.ny - 0.5f) * P.ly;
+  const float dz = P.lz / P.nz;
+  const float pz0 = ((z0 + 0.5f) / P.nz - 0.5f) * P.lz;
+  const unsigned view_bytes = P.pview2 * 4u;
+  const unsigned row_bytes = P.pnu2 * 4u;
+  const uint2 *rb = rects + (size_t)L * P.nviews;
+  // byte offset, within a rectangle, of the 4 elements each lane copies for
+  // this wave's copies (copy i = wv + 4 j covers LDS elements [256 i, 256 i +
+  // 256)); rows past the rectangle land in the buffer's unused tail
+  unsigned loff[(BPP_CHUNKS + 3) / 4];
 #pragma unroll
-    for (int o = 1; o <= 2; o <<= 1) {
-      ulo = min(ulo, __shfl_xor(ulo, o, 64));
-      uhi = max(uhi, __shfl_xor(uhi, o, 64));
-      vlo = min(vlo, __shfl_xor(vlo, o, 64));
-      vhi = max(vhi, __shfl_xor(vhi, o, 64));
-    }
-    BPSRect R;
-    R.u0 = __builtin_amdgcn_readfirstlane(max(ulo - 1, 0));
-    R.v0 = __builtin_amdgcn_readfirstlane(max(vlo - 1, 0));
-    R.W = __builtin_amdgcn_readfirstlane(min(uhi + 2, (int)P.pnu - 1)) - R.u0 + 1;
-    R.H = __builtin_amdgcn_readfirstlane(min(vhi + 2, (int)P.pnv - 1)) - R.v0 + 1;
-    R.rw = R.W <= 64 ? 64 : 128;
-    R.staged = R.W <= 128 && R.H * R.rw <= BPS_CAP;
-    return R;
-  };
-  // direct global-to-LDS copy of a rectangle (each instruction: 64 floats of one row)
-  auto issue = [&](int k, const BPSRect &R, int base) {
-    const __amdgpu_buffer_rsrc_t rs = rsrc_of(k);
-    for (int r = wv; r < R.H; r += 4)
-      for (int c0 = 0; c0 < R.W; c0 += 64)
+  for (int j = 0; j < (BPP_CHUNKS + 3) / 4; ++j) {
+    const int p = (wv + 4 * j) * 256 + ln * 4;
+    const int r = p / BPP_RW;
+    loff[j] = (unsigned)(r * (int)P.pnu2 + (p - r * BPP_RW)) * 4u;
+  }
+  auto issue = [&](int k, uint2 R, int base) {
+    // the rectangle's origin goes into the descriptor, so the hardware range
+    // check (offsets past the view read zero) covers the whole copy
+    const unsigned org = ((R.x >> 16) * P.pnu2 + (R.x & 0xffffu)) * 4u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)((const char *)(copy + (size_t)k * P.pview2) + org), (short)0, (int)(view_bytes - org),
+        0x00020000);
+    const int n = ((int)R.y * BPP_RW + 255) >> 8;
+#pragma unroll
+    for (int j = 0; j < (BPP_CHUNKS + 3) / 4; ++j)
+      if (wv + 4 * j < n)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rs, (__attribute__((address_space(3))) void *)&buf[base + r * R.rw + c0], 4,
-            (unsigned)(((R.v0 + r) * (int)P.pnu + R.u0 + c0 + ln) * 4), 0, 0, 0);
+            rs, (__attribute__((address_space(3))) void *)&buf[base + (wv + 4 * j) * 256], 16, loff[j], 0,
+            0, 0);
   };
   float acc[KZ];
 #pragma unroll
   for (int t = 0; t < KZ; ++t) acc[t] = 0.0f;
-  BPSRect cur = rect_of(0);
-  if (cur.staged) issue(0, cur, 0);
+  uint2 cur = rb[0];
+  uint2 nxt = rb[P.nviews > 1 ? 1 : 0];
+  if (cur.y) issue(0, cur, 0);
+  ViewF vw = views[0];
   for (int k = 0; k < P.nviews; ++k) {
-    const ViewF vw = views[k];
-    const __amdgpu_buffer_rsrc_t rsrc = rsrc_of(k);
+    // scalar loads one iteration ahead of their use
+    const uint2 nn = rb[k + 2 < P.nviews ? k + 2 : P.nviews - 1];
+    const ViewF vnext = views[k + 1 < P.nviews ? k + 1 : k];
     const float qx = fmaf(vw.c, px, vw.s * py);
     const float qy = fmaf(vw.c, py, -vw.s * px);
     const float mag = P.sdd * __builtin_amdgcn_rcpf(qy + P.sad);
@@ -270,37 +309,52 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bps(BPParams P, const ViewF *_
     // have, and nobody reads view k-1's buffer any more
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    BPSRect nxt{};
-    if (k + 1 < P.nviews) {
-      nxt = rect_of(k + 1);
-      if (nxt.staged) issue(k + 1, nxt, ((k + 1) & 1) * BPS_CAP);
-    }
-    const int base = (k & 1) * BPS_CAP;
-    const int du = iu - cur.u0;
-#pragma unroll
-    for (int t = 0; t < KZ; ++t) {
-      const float tv = __builtin_amdgcn_fmed3f(fmaf((float)t, dtv, tv0), 0.0f, P.clv);
-      const float av = __builtin_amdgcn_fractf(tv);
-      const int iv = (int)tv;
-      float a, bb, c, d;
-      if (cur.staged) {
-        const int q = base + (iv - cur.v0) * cur.rw + du;
-        a = buf[q];
-        bb = buf[q + 1];
-        c = buf[q + cur.rw];
-        d = buf[q + cur.rw + 1];
-      } else {
-        const unsigned o = ((unsigned)iv * P.pnu + (unsigned)iu) * 4u;
-        a = bld<float>(rsrc, o, 0);
-        bb = bld<float>(rsrc, o + 4, 0);
-        c = bld<float>(rsrc, o, P.pnu * 4);
-        d = bld<float>(rsrc, o + 4, P.pnu * 4);
-      }
-      const float r0 = fmaf(au, bb - a, a);
+    if (k + 1 < P.nviews && nxt.y) issue(k + 1, nxt, ((k + 1) & 1) * BPP_CAP);
+    auto tap = [&](int t, float a, float b, float c, float d, float av) {
+      const float r0 = fmaf(au, b - a, a);
       const float r1 = fmaf(au, d - c, c);
       acc[t] = fmaf(w, fmaf(av, r1 - r0, r0), acc[t]);
+    };
+    // the taps (iv, iu), (iv, iu+1), (iv+1, iu), (iv+1, iu+1) from elements
+    // at byte offsets o (row iv) and o + row (row iv+1)
+    auto taps = [&](int t, float av, auto ld, unsigned o, unsigned row) {
+      if constexpr (PAIRS) {
+        const unsigned w0 = ld(o), w1 = ld(o + row);
+        tap(t, __uint_as_float(w0 << 16), __uint_as_float(w0 & 0xffff0000u), __uint_as_float(w1 << 16),
+            __uint_as_float(w1 & 0xffff0000u), av);
+      } else {
+        tap(t, __uint_as_float(ld(o)), __uint_as_float(ld(o + 4)), __uint_as_float(ld(o + row)),
+            __uint_as_float(ld(o + row + 4)), av);
+      }
+    };
+    if (cur.y) {
+      // LDS byte address of element (iv, iu): iv * row + iu * 4 + s, with s
+      // uniform (readfirstlane keeps the compiler from splitting the sum)
+      const unsigned s = __builtin_amdgcn_readfirstlane(
+          (unsigned)(((k & 1) * BPP_CAP - (int)(cur.x & 0xffffu) - (int)(cur.x >> 16) * BPP_RW) * 4));
+      const unsigned colb = (unsigned)iu * 4u + s;
+      const char *lds = (const char *)buf;
+      auto ld = [&](unsigned o) { return *(const unsigned *)(lds + o); };
+#pragma unroll
+      for (int t = 0; t < KZ; ++t) {
+        const float tv = __builtin_amdgcn_fmed3f(fmaf((float)t, dtv, tv0), 0.0f, P.clv);
+        taps(t, __builtin_amdgcn_fractf(tv), ld, __umul24((unsigned)(int)tv, P.bpp_rowb) + colb,
+             BPP_RW * 4u);
+      }
+    } else {
+      const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(copy + (size_t)k * P.pview2), (short)0, (int)view_bytes, 0x00020000);
+      auto ld = [&](unsigned o) { return (unsigned)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)o, 0, 0); };
+      const unsigned ub = (unsigned)iu * 4u;
+#pragma unroll
+      for (int t = 0; t < KZ; ++t) {
+        const float tv = __builtin_amdgcn_fmed3f(fmaf((float)t, dtv, tv0), 0.0f, P.clv);
+        taps(t, __builtin_amdgcn_fractf(tv), ld, __umul24((unsigned)(int)tv, row_bytes) + ub, row_bytes);
+      }
     }
     cur = nxt;
+    nxt = nn;
+    vw = vnext;
   }
   if (!active) return;
 #pragma unroll
@@ -430,6 +484,34 @@ __global__ void __launch_bounds__(256) k_pad_plain(const TI *__restrict__ in, T 
   }
 }
 
+// Pair copy for k_bpp: dword (a, b) of a view holds the BF16 taps (a, b) (low
+// half) and (a+1, b) of the zero-bordered copy, i.e. the projection at
+// (u, v) = (a-1, b-1) and (a, b-1), zero outside; rows of pnu2 dwords.  Grid
+// as k_pad_quad over pnu2 columns.
+template <typename TI>
+__global__ void __launch_bounds__(256) k_pad_pair(const TI *__restrict__ in, unsigned *__restrict__ out,
+                                                  int nu, int nv, unsigned pnu2, unsigned pnv,
+                                                  int nviews) {
+  const int a = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+  if (a >= (int)pnu2) return;
+  const int b0 = (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), bs = (int)(gridDim.y * 4);
+  const int u = a - 1;
+  for (int k = (int)blockIdx.z; k < nviews; k += (int)gridDim.z) {
+    const TI *pk = in + (size_t)nu * nv * k;
+    unsigned *ok = out + (size_t)pnu2 * pnv * k;
+    for (int b = b0; b < (int)pnv; b += bs) {
+      const int v = b - 1;
+      unsigned lo = 0, hi = 0;
+      if (v >= 0 && v < nv) {
+        const TI *row = pk + (size_t)nu * v;
+        if (u >= 0 && u < nu) lo = to_storage<unsigned short>(row[u]);
+        if (u + 1 < nu) hi = to_storage<unsigned short>(row[u + 1]);
+      }
+      ok[(size_t)pnu2 * b + a] = lo | (hi << 16);
+    }
+  }
+}
+
 // Space carver (xray/conebeam_projection.cu:822-921): nearest-texel lookups,
 // z axis flipped, count of views with 0 != p < limit; mask = count < 3.
 __global__ void __launch_bounds__(256) k_spacecarver(BPParams P, int nu, int nv,
@@ -492,6 +574,9 @@ BPParams make_params(const cbct_geom *g, int nviews) {
   P.pnu = (unsigned)padded_dim(g->nu);
   P.pnv = (unsigned)padded_dim(g->nv);
   P.pview = (unsigned)(padded_dim(g->nu) * padded_dim(g->nv));
+  P.pnu2 = (P.pnu + 3u) & ~3u;
+  P.pview2 = P.pnu2 * P.pnv;
+  P.bpp_rowb = BPP_RW * 4;
   return P;
 }
 
@@ -505,6 +590,85 @@ inline int bp_kz(const cbct_ctx ctx, const cbct_geom *g) {
   if (ctx->bp_kz) return ctx->bp_kz;
   const size_t tiles16 = (size_t)ceil_div(g->nx, BP_TX) * ceil_div(g->ny, BP_TY) * ceil_div(g->nz, 16);
   return tiles16 < 1024 ? 8 : KZ;
+}
+
+// Whether the LDS-staged kernel should run: at least half of the blocks'
+// detector rectangles fit its LDS tile (staged block-views take about 0.7 of
+// the quad kernel's time; with a quarter staged the two were measured equal
+// at BF16), estimated on up to 8 evenly spaced views x a grid of up to 16 x
+// 16 block columns x every z-band, with one texel of slack on each side.
+// Cached per geometry, view range and z-depth until the view table changes.
+bool bpp_fits(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, int view0, int nviews, int kz) {
+  std::vector<float> key = {(float)g->nx, (float)g->ny, (float)g->nz, g->lx, g->ly, g->lz,
+                            (float)g->nu, (float)g->nv, g->fov_u, g->fov_v, g->sad, g->sdd,
+                            (float)view0, (float)nviews, (float)kz};
+  for (const auto &e : ctx->bpp_fit_cache)
+    if (e.first == key) return e.second;
+  const int gx = (int)ceil_div(g->nx, BP_TX), gy = (int)ceil_div(g->ny, BP_TY);
+  const int gzb = (int)ceil_div(g->nz, kz);
+  const float dz = P.lz / P.nz;
+  const int vstep = std::max(1, nviews / 8), sx = std::max(1, gx / 16), sy = std::max(1, gy / 16);
+  size_t total = 0, staged = 0;
+  for (int k = view0; k < view0 + nviews; k += vstep) {
+    const ViewF vw = ctx->views_h[k];
+    for (int by = 0; by < gy; by += sy)
+      for (int bx = 0; bx < gx; bx += sx) {
+        float ulo = 1e30f, uhi = -1e30f, mlo = 1e30f, mhi = -1e30f;
+        for (int c = 0; c < 4; ++c) {
+          const int xi = bx * BP_TX + ((c & 1) ? BP_TX - 1 : 0);
+          const int yi = by * BP_TY + ((c & 2) ? BP_TY - 1 : 0);
+          const float px = ((xi + 0.5f) / P.nx - 0.5f) * P.lx;
+          const float py = ((yi + 0.5f) / P.ny - 0.5f) * P.ly;
+          const float qx = vw.c * px + vw.s * py, qy = vw.c * py - vw.s * px;
+          const float mag = P.sdd / (qy + P.sad);
+          const float tu = std::min(std::max((qx * mag - vw.off_u) * P.su + P.cu, 0.0f), P.clu);
+          ulo = std::min(ulo, tu);
+          uhi = std::max(uhi, tu);
+          mlo = std::min(mlo, mag);
+          mhi = std::max(mhi, mag);
+        }
+        total += gzb;
+        const int u0 = std::max((int)std::floor(ulo) - 2, 0) & ~3;
+        const int W = std::min((int)std::floor(uhi) + 3, (int)P.pnu2 - 1) - u0 + 1;
+        if (W > BPP_RW) continue;
+        for (int zb = 0; zb < gzb; ++zb) {
+          // v is linear in pz * mag: extreme at the band's end slices and the
+          // extreme magnifications
+          const float pz0 = ((zb * kz + 0.5f) / P.nz - 0.5f) * P.lz, pz1 = pz0 + (kz - 1) * dz;
+          const float e[4] = {pz0 * mlo, pz0 * mhi, pz1 * mlo, pz1 * mhi};
+          const float plo = std::min(std::min(e[0], e[1]), std::min(e[2], e[3]));
+          const float phi = std::max(std::max(e[0], e[1]), std::max(e[2], e[3]));
+          const float vlo = std::min(std::max((plo - vw.off_v) * P.sv + P.cv, 0.0f), P.clv);
+          const float vhi = std::min(std::max((phi - vw.off_v) * P.sv + P.cv, 0.0f), P.clv);
+          const int v0 = std::max((int)std::floor(vlo) - 2, 0);
+          const int H = std::min((int)std::floor(vhi) + 3, (int)P.pnv - 1) - v0 + 1;
+          staged += H * BPP_RW <= BPP_CAP;
+        }
+      }
+  }
+  const bool fits = staged * 2 >= total;
+  if (ctx->bpp_fit_cache.size() >= 256) ctx->bpp_fit_cache.clear();
+  ctx->bpp_fit_cache.emplace_back(std::move(key), fits);
+  return fits;
+}
+
+// Whether the call runs the LDS-staged kernel (engine 2 where it pays,
+// engine 3 always; rectangle origins are packed in 16 bits).
+bool use_staged(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, int view0, int nviews, int kz) {
+  return ctx->bp_engine >= 2 && P.pnu2 < 65536 && P.pnv < 65536 &&
+         (ctx->bp_engine == 3 || bpp_fits(ctx, g, P, view0, nviews, kz));
+}
+
+template <bool PAIRS, int Z>
+void launch_bpp(const BPParams &P, dim3 grid, hipStream_t s, const ViewF *views, const unsigned *cp,
+                const uint2 *rects, float *vol, int fbp, int acc) {
+  const unsigned nblocks = grid.x * grid.y * grid.z;
+  if (fbp)
+    hipLaunchKernelGGL((k_bpp<PAIRS, true, Z>), dim3(nblocks), dim3(BP_TX, BP_TY), 0, s, P, views, cp,
+                       rects, vol, acc, grid.x, grid.y);
+  else
+    hipLaunchKernelGGL((k_bpp<PAIRS, false, Z>), dim3(nblocks), dim3(BP_TX, BP_TY), 0, s, P, views, cp,
+                       rects, vol, acc, grid.x, grid.y);
 }
 
 template <typename T, bool QUAD, int Z>
@@ -531,38 +695,42 @@ void launch_bp(const BPParams &P, dim3 grid, int kz, hipStream_t s, const ViewF 
 template <typename TI, typename T>
 int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int kz, int view0,
            int nviews, const TI *proj, float *vol, int fbp, int accumulate, hipStream_t s) {
-  // engine 2: the LDS-staged kernel on a plain copy for FP32 working copies
-  // (BF16 ones use quads, measured faster for them); engine 0's plain layout
-  // converts nothing: BF16 working copies use quads
-  if (ctx->bp_engine == 2 && sizeof(T) == 4) {
-    const size_t totalp = (size_t)P.pview * nviews;
-    int rc2 = cbct_reserve(ctx, ctx->ws->proj_pad, totalp * sizeof(T));
+  // the LDS-staged kernel on a pair copy for BF16 working copies, a plain
+  // one for FP32, after the pass that computes its rectangles
+  if (use_staged(ctx, g, P, view0, nviews, kz)) {
+    int rc2 = cbct_reserve(ctx, ctx->ws->proj_pad, (size_t)P.pview2 * nviews * 4);
     if (rc2) return rc2;
-    T *pp2 = static_cast<T *>(ctx->ws->proj_pad.ptr);
-    hipLaunchKernelGGL((k_pad_plain<TI, T>), pad_grid(P.pnu, nviews), dim3(256), 0, s, proj, pp2,
-                       g->nu, g->nv, P.pnu, P.pnv, nviews);
-    CBCT_LAUNCH_CHECK(ctx, "k_pad_plain");
-    cbct_prof_mark(ctx, PROF_BP, 0, s);
-    const unsigned nblocks = grid.x * grid.y * grid.z;
-    const float *pf = reinterpret_cast<const float *>(pp2);
+    const size_t nblocks = (size_t)grid.x * grid.y * grid.z;
+    rc2 = cbct_reserve(ctx, ctx->ws->bp_rects, nblocks * nviews * sizeof(uint2));
+    if (rc2) return rc2;
+    unsigned *cp = static_cast<unsigned *>(ctx->ws->proj_pad.ptr);
+    uint2 *rects = static_cast<uint2 *>(ctx->ws->bp_rects.ptr);
+    if (sizeof(T) == 2)
+      hipLaunchKernelGGL((k_pad_pair<TI>), pad_grid(P.pnu2, nviews), dim3(256), 0, s, proj, cp, g->nu,
+                         g->nv, P.pnu2, P.pnv, nviews);
+    else
+      hipLaunchKernelGGL((k_pad_plain<TI, float>), pad_grid(P.pnu2, nviews), dim3(256), 0, s, proj,
+                         reinterpret_cast<float *>(cp), g->nu, g->nv, P.pnu2, P.pnv, nviews);
+    CBCT_LAUNCH_CHECK(ctx, "k_pad_pair / k_pad_plain");
     const ViewF *vf = ctx->d_views_f + view0;
-    const dim3 blk(BP_TX, BP_TY);
-    if (kz == 8) {
-      if (fbp)
-        hipLaunchKernelGGL((k_bps<true, 8>), dim3(nblocks), blk, 0, s, P, vf, pf, vol, accumulate, grid.x, grid.y);
-      else
-        hipLaunchKernelGGL((k_bps<false, 8>), dim3(nblocks), blk, 0, s, P, vf, pf, vol, accumulate, grid.x, grid.y);
-    } else {
-      if (fbp)
-        hipLaunchKernelGGL((k_bps<true, KZ>), dim3(nblocks), blk, 0, s, P, vf, pf, vol, accumulate, grid.x, grid.y);
-      else
-        hipLaunchKernelGGL((k_bps<false, KZ>), dim3(nblocks), blk, 0, s, P, vf, pf, vol, accumulate, grid.x, grid.y);
-    }
+    const size_t nr = nblocks * nviews;
+    if (kz == 8)
+      hipLaunchKernelGGL((k_bpp_rects<8>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, P, vf, rects,
+                         grid.x, grid.y, (unsigned)nblocks);
+    else
+      hipLaunchKernelGGL((k_bpp_rects<KZ>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, P, vf, rects,
+                         grid.x, grid.y, (unsigned)nblocks);
+    CBCT_LAUNCH_CHECK(ctx, "k_bpp_rects");
+    cbct_prof_mark(ctx, PROF_BP, 0, s);
+    if (kz == 8) launch_bpp<sizeof(T) == 2, 8>(P, grid, s, vf, cp, rects, vol, fbp, accumulate);
+    else launch_bpp<sizeof(T) == 2, KZ>(P, grid, s, vf, cp, rects, vol, fbp, accumulate);
     cbct_prof_mark(ctx, PROF_BP, 1, s);
-    CBCT_LAUNCH_CHECK(ctx, "k_bps");
+    CBCT_LAUNCH_CHECK(ctx, "k_bpp");
     return CBCT_OK;
   }
-  const bool quad = ctx->bp_engine == 1 || sizeof(TI) != sizeof(T);
+  // engine 0's plain layout converts nothing: BF16 working copies of FP32
+  // projections use quads
+  const bool quad = ctx->bp_engine >= 1 || sizeof(TI) != sizeof(T);
   const unsigned pnu = (unsigned)padded_dim(g->nu), pnv = (unsigned)padded_dim(g->nv);
   const size_t total = (size_t)P.pview * nviews;
   int rc = cbct_reserve(ctx, ctx->ws->proj_pad, total * sizeof(T) * (quad ? 4 : 1));
@@ -632,6 +800,14 @@ int cbct_bp(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, const void 
                               vol, fbp, accumulate, s);
 }
 
+const char *cbct_bp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews) {
+  if (!ctx || !g || cbct_check_geom(ctx, g, "cbct_bp_kernel") ||
+      cbct_check_views(ctx, view0, nviews, "cbct_bp_kernel") || nviews == 0)
+    return "";
+  const BPParams P = make_params(g, nviews);
+  return use_staged(ctx, g, P, view0, nviews, bp_kz(ctx, g)) ? "k_bpp" : "k_bp";
+}
+
 int cbct_storage(cbct_ctx ctx, int dtype) {
   if (!ctx) return CBCT_ERR_ARG;
   if (dtype != CBCT_F32 && dtype != CBCT_BF16)
@@ -642,7 +818,7 @@ int cbct_storage(cbct_ctx ctx, int dtype) {
 
 int cbct_bp_engine(cbct_ctx ctx, int engine) {
   if (!ctx) return CBCT_ERR_ARG;
-  if (engine < 0 || engine > 2)
+  if (engine < 0 || engine > 3)
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_bp_engine: engine %d not available", engine);
   ctx->bp_engine = engine;
   return CBCT_OK;

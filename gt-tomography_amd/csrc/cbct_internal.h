@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "cbct.h"
@@ -36,6 +37,7 @@ struct Workspace {
   DeviceBuffer red;         // reduction partials
   DeviceBuffer filt_tmp;    // filter staging
   DeviceBuffer ssim_ws;     // MS-SSIM moment fields and taps
+  DeviceBuffer bp_rects;    // backprojector: per (block, view) detector rectangles
   hipEvent_t done = nullptr;
   hipStream_t last = nullptr;   // stream of the last call (compared by value only)
   bool recorded = false;
@@ -50,6 +52,7 @@ struct cbct_ctx_s {
   int nviews = 0;
   std::vector<float> angles_deg;
   std::vector<float> offsets;  // interleaved (u, v)
+  std::vector<ViewF> views_h;  // host copy of the float view table
   ViewD *d_views_d = nullptr;
   ViewF *d_views_f = nullptr;
   float *d_angles = nullptr;  // degrees, for the Parker weights
@@ -61,7 +64,11 @@ struct cbct_ctx_s {
   int filter_nu = 0;
   int filter_engine = 1;
   int fp_engine = 1;             // 1: detector-column sharing, 0: one thread per ray
-  int bp_engine = 2;             // 2: LDS-staged (FP32 copies) / quad (BF16), 1: quad, 0: plain
+  int bp_engine = 2;             // 2: LDS-staged, 3: LDS-staged forced (BF16), 1: quad, 0: plain
+  // backprojector: whether the LDS tile of the BF16 pair kernel holds the
+  // blocks' detector rectangles, per (geometry, views, z-depth); cleared
+  // with the view table
+  std::vector<std::pair<std::vector<float>, bool>> bpp_fit_cache;
   int bp_kz = 0;                 // z-voxels per BP thread: 0 = chosen by size, else 8 or 16
   int fp_max_rpl = 8;            // column engine: most detector rows per lane
   int fp_view_group = 6;         // column engine: adjacent views launched together per column

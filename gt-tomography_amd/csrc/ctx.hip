@@ -73,7 +73,7 @@ int cbct_check_geom(cbct_ctx ctx, const cbct_geom *g, const char *who) {
 }
 
 void Workspace::free_buffers() {
-  DeviceBuffer *bufs[] = {&vol_pad, &proj_pad, &red, &filt_tmp, &ssim_ws};
+  DeviceBuffer *bufs[] = {&vol_pad, &proj_pad, &red, &filt_tmp, &ssim_ws, &bp_rects};
   for (auto *b : bufs) {
     if (b->ptr) (void)hipFree(b->ptr);
     b->ptr = nullptr;
@@ -268,7 +268,8 @@ const char *cbct_last_error(cbct_ctx ctx) {
 size_t cbct_workspace_bytes(cbct_ctx ctx) {
   if (!ctx) return 0;
   const Workspace &w = *ctx->ws;
-  return w.vol_pad.bytes + w.proj_pad.bytes + w.red.bytes + w.filt_tmp.bytes + w.ssim_ws.bytes;
+  return w.vol_pad.bytes + w.proj_pad.bytes + w.red.bytes + w.filt_tmp.bytes + w.ssim_ws.bytes +
+         w.bp_rects.bytes;
 }
 
 int cbct_views_set(cbct_ctx ctx, const float *angles_deg, const float *offsets, int n) {
@@ -310,6 +311,8 @@ int cbct_views_set(cbct_ctx ctx, const float *angles_deg, const float *offsets, 
   if ((rc = cbct_upload(ctx, ctx->d_views_f, vf.data(), sizeof(ViewF) * n))) return rc;
   if ((rc = cbct_upload(ctx, ctx->d_angles, angles_deg, sizeof(float) * n))) return rc;
   ctx->nviews = n;
+  ctx->views_h = std::move(vf);
+  ctx->bpp_fit_cache.clear();
   // the event that marks the end of the calls reading this table
   if (!ctx->done) CBCT_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming));
   return CBCT_OK;

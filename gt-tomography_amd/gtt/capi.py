@@ -111,6 +111,7 @@ def load():
         "cbct_fp_tile": (I, [P, I]),
         "cbct_fp_view_group": (I, [P, I]),
         "cbct_bp_engine": (I, [P, I]),
+        "cbct_bp_kernel": (ctypes.c_char_p, [P, G, I, I]),
         "cbct_bp_kz": (I, [P, I]),
         "cbct_storage": (I, [P, I]),
         "cbct_spacecarver": (I, [P, G, I, I, P, P, F, P]),
@@ -280,6 +281,11 @@ class Context:
         return self.lib.cbct_fp_kernel(self.h, ctypes.byref(g), view0, n, code,
                                        samples_per_pixel).decode()
 
+    def bp_kernel(self, g: Geom, view0=0, nviews=None):
+        """Name of the kernel bp() would launch (for profiler output)."""
+        n = self.nviews() - view0 if nviews is None else nviews
+        return self.lib.cbct_bp_kernel(self.h, ctypes.byref(g), view0, n).decode()
+
     def fp_autotune(self, g: Geom, view0=0, nviews=None, dtype="f32", samples_per_pixel=1.5,
                     retune=False, stream=None):
         """Time the column-engine candidates for this geometry (unless it was
@@ -304,8 +310,9 @@ class Context:
         self._check(self.lib.cbct_fp_view_group(self.h, int(views)))
 
     def bp_engine(self, engine):
-        """2 = default (LDS-staged kernel for FP32 copies, else as 1), 1 = quad-texel
-        projection copy, 0 = plain padded copy."""
+        """2 = default (LDS-staged kernel where its tile holds the blocks'
+        detector rectangles, else as 1), 3 = LDS-staged kernel always, 1 =
+        quad-texel projection copy, 0 = plain padded copy."""
         self._check(self.lib.cbct_bp_engine(self.h, int(engine)))
 
     def bp_kz(self, kz):

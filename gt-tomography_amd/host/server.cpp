@@ -78,9 +78,13 @@ size_t footprint_bytes(const gtt_request &r) {
   const size_t nvox = (size_t)r.geom.nx * r.geom.ny * r.geom.nz;
   const size_t nproj = (size_t)(r.geom.nu + 3) * (r.geom.nv + 3) * (size_t)r.nviews;
   const size_t slack = size_t(64) << 20;
-  if (r.kind == GTT_REQ_FDK) return 4 * (nvox * 23 / 10 + 8 * nproj) + slack;
+  // the LDS-staged backprojector's rectangle table: 8 bytes per 16 x 16 x 8
+  // (or x 16) voxel block and view of one backprojection call
+  const size_t blocks = (size_t)((r.geom.nx + 15) / 16) * ((r.geom.ny + 15) / 16) * ((r.geom.nz + 7) / 8);
+  if (r.kind == GTT_REQ_FDK) return 4 * (nvox * 23 / 10 + 8 * nproj) + 8 * blocks * (size_t)r.nviews + slack;
   const size_t S = r.subsets > 0 ? (size_t)r.subsets : 1;
-  return 4 * (nvox * (10 * S + 33) / 10 + 4 * nproj + 4 * ((nproj + S - 1) / S)) + slack;
+  return 4 * (nvox * (10 * S + 33) / 10 + 4 * nproj + 4 * ((nproj + S - 1) / S)) +
+         8 * blocks * (((size_t)r.nviews + S - 1) / S) + slack;
 }
 
 // The request's storage type is the lane's default while its operators are

@@ -178,10 +178,14 @@ int cbct_fp_tile(cbct_ctx ctx, int max_rows_per_lane);
  * No reference counterpart. */
 int cbct_fp_view_group(cbct_ctx ctx, int views);
 
-/* Backprojector engine: 2 = default: with FP32 working copies, the
- * LDS-staged kernel (per view, each block's detector rectangle is copied from
- * a plain zero-bordered copy into LDS by direct global-to-LDS loads, one view
- * ahead, and read from there), with BF16 working copies as 1; 1 = quad-texel
+/* Backprojector engine: 2 = default: LDS-staged kernels (per view, each
+ * block's detector rectangle is copied into LDS by direct global-to-LDS
+ * loads, one view ahead, and read from there): with FP32 working copies from
+ * a plain zero-bordered copy; with BF16 working copies from a copy of
+ * horizontal tap pairs (two taps per dword, one LDS read per voxel-view),
+ * where the rectangles fit the LDS tile at a sample of the views, else as 1;
+ * 3 = as 2, with the BF16 pair kernel whatever the rectangles' size (views
+ * whose rectangle does not fit read the pair copy directly); 1 = quad-texel
  * copy of the projections (the four bilinear taps of a voxel-view in one
  * 16-byte (FP32) or 8-byte (BF16) load, 4x the padded projection workspace);
  * 0 = plain zero-bordered copy, taps loaded one by one.  All give
@@ -189,9 +193,16 @@ int cbct_fp_view_group(cbct_ctx ctx, int views);
  * a layered texture). */
 int cbct_bp_engine(cbct_ctx ctx, int engine);
 
+/* Name of the backprojection kernel cbct_bp would launch for these
+ * arguments under the context's current settings ("k_bpp" or "k_bp", either
+ * storage type; "" on bad arguments), for matching profiler output.  No
+ * reference counterpart. */
+const char *cbct_bp_kernel(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews);
+
 /* Backprojector: z-voxels per thread, 0 = chosen by size (16, or 8 when 16
  * would make fewer than 1024 tiles), or 8 / 16 forced (tests run both at
- * every size).  Results do not depend on it.  No reference counterpart. */
+ * every size).  The two differ in rounding only (each thread steps the
+ * detector row coordinate from its first voxel).  No reference counterpart. */
 int cbct_bp_kz(cbct_ctx ctx, int kz);
 
 /* Storage type of the projectors' working copies (the zero-bordered volume

@@ -89,14 +89,18 @@ def test_request_footprint_and_server_without_gpu():
     req = S.Request(kind=S.REQ_OS_SART, geom=g, angles=spec.angles(),
                     projections=np.zeros(g.proj_shape(30), np.float32))
     nvox, nproj = 40 ** 3, (64 + 3) * (48 + 3) * 30
+    blocks = 3 * 3 * 5                                  # 16 x 16 x 8 voxel blocks
     # OS-SART: S + 2 volumes, the shared padded volume copy (<= 1.3), four
-    # sinograms and one subset's quad-texel copy; FDK: 2.3 volumes, 8 sinograms
+    # sinograms, one subset's quad-texel copy and the backprojector's
+    # rectangle table for one subset; FDK: 2.3 volumes, 8 sinograms, the
+    # rectangle table for all views
     for S_ in (1, 5, 12):
         req.subsets = S_
-        want = 4 * (nvox * (10 * S_ + 33) // 10 + 4 * nproj + 4 * (-(-nproj // S_))) + (64 << 20)
+        want = (4 * (nvox * (10 * S_ + 33) // 10 + 4 * nproj + 4 * (-(-nproj // S_)))
+                + 8 * blocks * (-(-30 // S_)) + (64 << 20))
         assert req.footprint() == want
     req.kind = S.REQ_FDK
-    assert req.footprint() == 4 * (nvox * 23 // 10 + 8 * nproj) + (64 << 20)
+    assert req.footprint() == 4 * (nvox * 23 // 10 + 8 * nproj) + 8 * blocks * 30 + (64 << 20)
     try:
         import torch
         if torch.cuda.is_available():

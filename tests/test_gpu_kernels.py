@@ -44,6 +44,10 @@ CASES = {
     "cube32": (dict(nx=32, ny=32, nz=32, nu=48, nv=40), 24, False),
     "ragged": (dict(nx=23, ny=19, nz=17, nu=37, nv=29, vol_mm=(23.0, 19.0, 25.5)), 13, True),
     "flat": (dict(nx=40, ny=36, nz=6, nu=64, nv=12), 18, False),
+    # ~6 detector texels per voxel: half of the 8-deep blocks' detector
+    # rectangles (and none of the 16-deep ones) fit the pair kernel's LDS
+    # tile, so its staged and direct paths alternate between views
+    "zoom": (dict(nx=20, ny=20, nz=16, nu=100, nv=80), 24, True),
 }
 
 
@@ -204,7 +208,8 @@ def test_bp_parity(ctx, name, fbp, kz):
 @pytest.mark.parametrize("name", list(CASES))
 def test_bp_engines_bitwise_identical(ctx, name, dtype):
     """Quad-texel and plain padded projection copies, and the LDS-staged
-    kernel, give identical bits at 8 and at 16 z-voxels per thread (FBP and
+    kernels (engine 3 forces the BF16 pair kernel where engine 2 would pick
+    quads), give identical bits at 8 and at 16 z-voxels per thread (FBP and
     plain, FP32 and BF16 storage, accumulating into a volume).  The two
     z-depths differ in rounding: each thread steps v from its first voxel."""
     g, angles, offsets = make_case(name)
@@ -218,7 +223,7 @@ def test_bp_engines_bitwise_identical(ctx, name, dtype):
             for kz in (8, 16):
                 ctx.bp_kz(kz)
                 outs = []
-                for e in (0, 1, 2):
+                for e in (0, 1, 2, 3):
                     ctx.bp_engine(e)
                     out = start.clone()
                     ctx.bp(g, dev(proj, tdt), out, fbp=fbp, accumulate=True)
