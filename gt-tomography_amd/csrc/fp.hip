@@ -747,7 +747,10 @@ __global__ void __launch_bounds__(64) k_fp_col(FPParams P, const ViewD *__restri
 }
 
 // The cached variant, held to five waves per SIMD (five measured faster than
-// four with more registers, and than six with BF16 copies).
+// four with more registers, and than six with BF16 copies).  With FP32
+// copies and 6 rows per lane, five waves spill 3 VGPRs to scratch; four
+// waves without the spill measured slower (162 against 143 ms at 512^3 x
+// 720 views).
 template <typename T, int RPL, int NCH>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 k_fp_colc(FPParams P, const ViewD *__restrict__ views, const T *__restrict__ vt,

@@ -276,21 +276,21 @@ class Context:
 
     def fp_kernel(self, g: Geom, view0=0, nviews=None, dtype="f32", samples_per_pixel=1.5):
         """Name of the kernel fp() would launch (for profiler output)."""
-        n = self.nviews() - view0 if nviews is None else nviews
+        n = self.nviews - view0 if nviews is None else nviews
         code = CBCT_BF16 if dtype in ("bf16", "bfloat16") else CBCT_F32
         return self.lib.cbct_fp_kernel(self.h, ctypes.byref(g), view0, n, code,
                                        samples_per_pixel).decode()
 
     def bp_kernel(self, g: Geom, view0=0, nviews=None):
         """Name of the kernel bp() would launch (for profiler output)."""
-        n = self.nviews() - view0 if nviews is None else nviews
+        n = self.nviews - view0 if nviews is None else nviews
         return self.lib.cbct_bp_kernel(self.h, ctypes.byref(g), view0, n).decode()
 
     def fp_autotune(self, g: Geom, view0=0, nviews=None, dtype="f32", samples_per_pixel=1.5,
                     retune=False, stream=None):
         """Time the column-engine candidates for this geometry (unless it was
         tuned before); engine 1 then uses the fastest.  Returns its time in ms."""
-        n = self.nviews() - view0 if nviews is None else nviews
+        n = self.nviews - view0 if nviews is None else nviews
         code = CBCT_BF16 if dtype in ("bf16", "bfloat16") else CBCT_F32
         out = ctypes.c_double()
         self._check(self.lib.cbct_fp_autotune(self.h, ctypes.byref(g), view0, n, code, samples_per_pixel,

@@ -204,6 +204,29 @@ def test_bp_parity(ctx, name, fbp, kz):
     assert err <= TOL
 
 
+def test_kernel_names(ctx):
+    """The kernel-name queries follow the engine settings (all views by
+    default): the staged backprojector where its tile holds the rectangles
+    (cube32), quads where it does not (zoom at 16 z-voxels per thread) or
+    when engine 1 is set, the staged kernel whenever engine 3 is set."""
+    g, angles, offsets = make_case("cube32")
+    ctx.set_views(angles, offsets)
+    assert ctx.fp_kernel(g) in ("k_fp_ray", "k_fp_col", "k_fp_colc")
+    assert ctx.bp_kernel(g) == "k_bpp"
+    gz, az, oz = make_case("zoom")
+    ctx.set_views(az, oz)
+    try:
+        ctx.bp_kz(16)
+        assert ctx.bp_kernel(gz) == "k_bp"
+        ctx.bp_engine(3)
+        assert ctx.bp_kernel(gz) == "k_bpp"
+        ctx.bp_engine(1)
+        assert ctx.bp_kernel(gz, 0, 4) == "k_bp"
+    finally:
+        ctx.bp_engine(2)
+        ctx.bp_kz(0)
+
+
 @pytest.mark.parametrize("dtype", ["f32", "bf16"])
 @pytest.mark.parametrize("name", list(CASES))
 def test_bp_engines_bitwise_identical(ctx, name, dtype):
