@@ -158,20 +158,37 @@ def kernel_rooflines(fp_kernel, fp_ms, bp_kernel, bp_ms, workload, nvox, nviews,
             fp["hbm_frac_measured"] = round(fp["hbm_gbs_measured"] / HBM_PEAK_GBS, 4)
     out["fp"] = fp
     pmc = read_pmc(bp_kernel, workload, BP_SOURCES)
-    bp = {"kernel": bp_kernel, "kernel_ms": round(bp_ms, 3), "bound": "hbm", "peak": HBM_PEAK_GBS,
-          "unit": "GB/s",
+    bp = {"kernel": bp_kernel, "kernel_ms": round(bp_ms, 3),
           "effective_gbs_footprint": round(footprint_bytes("bp", nvox, nviews, nrays, nsamp, s_store,
                                                            s_store) / (bp_ms / 1e3) / 1e9, 1)}
     if pmc:
+        # the binding resource: measured HBM bytes or VALU issue, whichever
+        # runs closer to its peak
         c = pmc["counters_per_launch"]
+        hbm = valu = None
         if "bytes_per_launch" in pmc:
+            gbs = pmc["bytes_per_launch"] / (bp_ms / 1e3) / 1e9
+            hbm = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
             bp["traffic"] = pmc["bytes_per_launch"]
-            bp["achieved"] = round(pmc["bytes_per_launch"] / (bp_ms / 1e3) / 1e9, 1)
-            bp["frac"] = round(bp["achieved"] / HBM_PEAK_GBS, 4)
+            bp["hbm_gbs_measured"] = hbm["achieved"]
+            bp["hbm_frac_measured"] = hbm["frac"]
             # compulsory: the sinogram once at storage precision, the volume written once
             bp["traffic_vs_compulsory"] = round(pmc["bytes_per_launch"] / (nrays * s_store + nvox * 4), 2)
         if "SQ_INSTS_VALU" in c:
-            bp["valu_frac"] = round(c["SQ_INSTS_VALU"] * 64 / (bp_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)
+            lane_ops = c["SQ_INSTS_VALU"] * 64
+            tops = lane_ops / (bp_ms / 1e3) / 1e12
+            valu = {"achieved": round(tops, 2), "peak": VALU_PEAK_TOPS, "unit": "T VALU lane-instructions/s",
+                    "frac": round(tops / VALU_PEAK_TOPS, 4)}
+            bp["valu_frac"] = valu["frac"]
+            bp["valu_lane_ops_per_update"] = round(lane_ops / (nvox * nviews), 2)
+        best = max((r for r in (("hbm", hbm), ("valu", valu)) if r[1]), key=lambda r: r[1]["frac"],
+                   default=None)
+        if best:
+            bp["bound"] = best[0]
+            bp.update(best[1])
+    else:
+        bp.update({"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"})
     out["bp"] = bp
     if filt:
         e1 = filt["engine1"]
