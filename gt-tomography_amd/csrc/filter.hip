@@ -165,9 +165,9 @@ __global__ void __launch_bounds__(256) k_toeplitz_f32(const float *__restrict__ 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int MB = 128, NB = 128, KB = 32;
-constexpr int SEGV = KB / 16;   // 16-byte vectors per thread and tile row half
-constexpr int LDK = KB + 8;  // LDS row pitch in halves (16-byte aligned rows, staggered banks)
+// Block tile MB x NB, K-steps of KB (one LDS row of a part = KB halves =
+// 64 bytes = four 16-byte chunks).
+constexpr int MB = 256, NB = 256, KB = 32;
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -176,12 +176,26 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // One wave per row: weights (cosine plane, Parker), per-row power-of-two
-// scale putting max|x| in [2^13, 2^14), then the fp16 hi/lo split.
+// scale putting max|x| in [2^13, 2^14), then the fp16 hi/lo split.  Rows of
+// up to SPLIT_V x 256 elements (vec: nu a multiple of 4, the input 16-byte
+// aligned) stay in registers between the max and the split (16-byte loads,
+// 8-byte stores); others are read twice, one element per lane and step.
+constexpr int SPLIT_V = 8;
+
+// x[u] with its weights, multiplied in the order of the FP32 chain
+__device__ __forceinline__ float weighted(const float *x, const float *cw, int parker, int u, int nu,
+                                          float beta, float delta, float tan_delta) {
+  float v = x[u];
+  if (cw) v *= cw[u];
+  if (parker) v *= parker_weight(u, nu, beta, delta, tan_delta);
+  return v;
+}
+
 __global__ void __launch_bounds__(256) k_split_rows(const float *__restrict__ in,
                                                     const float *__restrict__ cosw, int nu, int nv,
                                                     size_t nrows, size_t mp, int kp,
                                                     const float *__restrict__ angles, int parker,
-                                                    float delta, float tan_delta,
+                                                    float delta, float tan_delta, int vec,
                                                     _Float16 *__restrict__ xh,
                                                     _Float16 *__restrict__ xl,
                                                     float *__restrict__ rinv) {
@@ -197,46 +211,91 @@ __global__ void __launch_bounds__(256) k_split_rows(const float *__restrict__ in
   const float *x = in + r * nu;
   const float *cw = cosw ? cosw + (r % nv) * nu : nullptr;
   const float beta = parker ? angles[r / nv] * (3.14159265358979f / 180.0f) : 0.0f;
+  auto scale_of = [](float mx, float &sc, float &inv) {
+    sc = inv = 1.0f;
+    if (mx > 0.0f && mx <= 3.4e38f) {
+      int ex;
+      (void)frexpf(mx, &ex);               // mx = f * 2^ex, f in [0.5, 1)
+      const int s = min(14 - ex, 126);
+      sc = ldexpf(1.0f, s);
+      inv = ldexpf(1.0f, -s);
+    }
+  };
+  auto split = [](float v, _Float16 &h, _Float16 &l) {
+    h = (_Float16)v;
+    l = (_Float16)(v - (float)h);
+  };
+  if (vec) {
+    float4 v[SPLIT_V];
+    float mx = 0.0f;
+#pragma unroll
+    for (int i = 0; i < SPLIT_V; ++i) {
+      const int u = 4 * lane + 256 * i;
+      v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (u < nu) {
+        v[i] = *reinterpret_cast<const float4 *>(x + u);
+        if (cw) {
+          const float4 c = *reinterpret_cast<const float4 *>(cw + u);
+          v[i].x *= c.x; v[i].y *= c.y; v[i].z *= c.z; v[i].w *= c.w;
+        }
+        if (parker) {
+          v[i].x *= parker_weight(u, nu, beta, delta, tan_delta);
+          v[i].y *= parker_weight(u + 1, nu, beta, delta, tan_delta);
+          v[i].z *= parker_weight(u + 2, nu, beta, delta, tan_delta);
+          v[i].w *= parker_weight(u + 3, nu, beta, delta, tan_delta);
+        }
+        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[i].x), fabsf(v[i].y)), fmaxf(fabsf(v[i].z), fabsf(v[i].w))));
+      }
+    }
+    mx = wave_max(mx);
+    float sc, inv;
+    scale_of(mx, sc, inv);
+#pragma unroll
+    for (int i = 0; i < SPLIT_V; ++i) {
+      const int u = 4 * lane + 256 * i;
+      if (u < kp) {   // kp - nu < 32 and both are multiples of 4: whole groups
+        _Float16 h[4], l[4];
+        split(v[i].x * sc, h[0], l[0]);
+        split(v[i].y * sc, h[1], l[1]);
+        split(v[i].z * sc, h[2], l[2]);
+        split(v[i].w * sc, h[3], l[3]);
+        typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<half4 *>(oh + u) = half4{h[0], h[1], h[2], h[3]};
+        *reinterpret_cast<half4 *>(ol + u) = half4{l[0], l[1], l[2], l[3]};
+      }
+    }
+    if (lane == 0) rinv[r] = inv;
+    return;
+  }
   float mx = 0.0f;
-  for (int u = lane; u < nu; u += 64) {
-    float v = x[u];
-    if (cw) v *= cw[u];
-    if (parker) v *= parker_weight(u, nu, beta, delta, tan_delta);
-    mx = fmaxf(mx, fabsf(v));
-  }
+  for (int u = lane; u < nu; u += 64)
+    mx = fmaxf(mx, fabsf(weighted(x, cw, parker, u, nu, beta, delta, tan_delta)));
   mx = wave_max(mx);
-  float sc = 1.0f, inv = 1.0f;
-  if (mx > 0.0f && mx <= 3.4e38f) {
-    int ex;
-    (void)frexpf(mx, &ex);               // mx = f * 2^ex, f in [0.5, 1)
-    const int s = min(14 - ex, 126);
-    sc = ldexpf(1.0f, s);
-    inv = ldexpf(1.0f, -s);
-  }
+  float sc, inv;
+  scale_of(mx, sc, inv);
   for (int u = lane; u < kp; u += 64) {
     float v = 0.0f;
-    if (u < nu) {
-      v = x[u];
-      if (cw) v *= cw[u];
-      if (parker) v *= parker_weight(u, nu, beta, delta, tan_delta);
-      v *= sc;
-    }
-    const _Float16 h = (_Float16)v;
+    if (u < nu) v = weighted(x, cw, parker, u, nu, beta, delta, tan_delta) * sc;
+    _Float16 h, l;
+    split(v, h, l);
     oh[u] = h;
-    ol[u] = (_Float16)(v - (float)h);
+    ol[u] = l;
   }
   if (lane == 0) rinv[r] = inv;
 }
 
-// 256 threads: four waves in a 2x2 arrangement, each owning a 64x64 output
-// block as 2x2 tiles of the 32x32x16 fp16 MFMA.  K advances in steps of 32
-// through LDS; the next step's global loads are issued before the MFMAs.
-// At most two waves per SIMD: with the register budget of more, the compiler
-// put part of the kernel's state in scratch memory (half the speed).  The
-// prefetched slices are named registers: as a private array the compiler
-// moved them to LDS (72 KB of LDS per block instead of 40, 6.5 ms instead of
-// 4.45 at 552,960 rows of 1024).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) k_toeplitz_mfma(const _Float16 *__restrict__ xh,
+// 512 threads: eight waves in a 2 x 4 arrangement, each owning a 128 x 64
+// output block as 4 x 2 tiles of the 32x32x16 fp16 MFMA.  Per K-step the
+// block's 256 x 32 row slice and 256 x 32 Toeplitz slice (hi and lo parts)
+// go straight from global memory into LDS (16-byte direct loads, no
+// registers), double-buffered: the next step's copies are issued before this
+// step's products, one barrier per step.  LDS rows are 64 bytes; chunk c of
+// row r is stored at chunk c ^ ((r >> 1) & 3), so the 8 rows a group of 8
+// lanes reads at one chunk fall in 8 different bank groups, and a direct load
+// (64 lanes -> 1 KB contiguous) still fills whole rows.  One block (two
+// waves per SIMD) per CU.  The products accumulate in the order of K, hi*hi,
+// hi*lo, lo*hi per 16 of K.
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) k_toeplitz_mfma(const _Float16 *__restrict__ xh,
                                                        const _Float16 *__restrict__ xl,
                                                        const float *__restrict__ rinv,
                                                        const _Float16 *__restrict__ th,
@@ -244,62 +303,78 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
                                                        float *__restrict__ Y, size_t nrows, int nu,
                                                        int kp, unsigned ntn, unsigned nblocks,
                                                        float inv_st) {
-  __shared__ __attribute__((aligned(16))) _Float16 sa[2][MB * LDK];
-  __shared__ __attribute__((aligned(16))) _Float16 sb[2][NB * LDK];
+  // [stage][part][row][KB halves]: rows of the block's X slice, rows (n) of T's
+  __shared__ __attribute__((aligned(1024))) _Float16 sa[2][2][MB * KB];
+  __shared__ __attribute__((aligned(1024))) _Float16 sb[2][2][NB * KB];
   const unsigned L = xcd_remap(blockIdx.x, nblocks);
   const unsigned tn = L % ntn, tm = L / ntn;
   const size_t m0 = (size_t)tm * MB;
   const int n0 = (int)tn * NB;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  // loader: thread -> (row, half of the KB columns) of the 128 x KB tiles
-  const int lr = tid >> 1, lc = (tid & 1) * (KB / 2);
-  const uint4 *gah = reinterpret_cast<const uint4 *>(xh + (m0 + lr) * kp + lc);
-  const uint4 *gal = reinterpret_cast<const uint4 *>(xl + (m0 + lr) * kp + lc);
-  const uint4 *gbh = reinterpret_cast<const uint4 *>(th + (size_t)(n0 + lr) * kp + lc);
-  const uint4 *gbl = reinterpret_cast<const uint4 *>(tl + (size_t)(n0 + lr) * kp + lc);
-  // the next K-step's tile slices, in registers (named, so the compiler
-  // does not move them to LDS as it does a private array)
-  static_assert(SEGV == 2, "two 16-byte slices per thread and part");
-  uint4 rah0 = gah[0], rah1 = gah[1], ral0 = gal[0], ral1 = gal[1];
-  uint4 rbh0 = gbh[0], rbh1 = gbh[1], rbl0 = gbl[0], rbl1 = gbl[1];
-  floatx16 acc[2][2];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave >> 2, wn = wave & 3;
+  // direct loads: a 1 KB copy fills 16 rows; lane -> row lane / 4 of them,
+  // stored chunk lane & 3, holding global chunk (lane & 3) ^ ((row >> 1) & 3)
+  // (16-row groups start at even rows, so (row >> 1) & 3 depends on the lane only)
+  const int crow = lane >> 2, cchunk = (lane & 3) ^ ((crow >> 1) & 3);
+  const size_t rowb = (size_t)kp * sizeof(_Float16);
+  const int coff = crow * (int)rowb + cchunk * 16;   // bytes, + 16 rows per copy
+  auto rsrc = [](const _Float16 *p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+  };
+  const __amdgpu_buffer_rsrc_t rah = rsrc(xh + m0 * kp, MB * rowb), ral = rsrc(xl + m0 * kp, MB * rowb);
+  const __amdgpu_buffer_rsrc_t rbh = rsrc(th + (size_t)n0 * kp, NB * rowb),
+                               rbl = rsrc(tl + (size_t)n0 * kp, NB * rowb);
+  typedef __attribute__((address_space(3))) void *lds_ptr;
+  // wave w copies 16-row groups w and w + 8 of each part of both slices
+  auto stage = [&](int k0, int st) {
+    const int kb = k0 * (int)sizeof(_Float16);
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int j = 0; j < 2; ++j) {
+      const int q = wave + 8 * j;
+      const int vo = coff + q * 16 * (int)rowb;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rah, (lds_ptr)&sa[st][0][q * 16 * KB], 16, vo, kb, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(ral, (lds_ptr)&sa[st][1][q * 16 * KB], 16, vo, kb, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rbh, (lds_ptr)&sb[st][0][q * 16 * KB], 16, vo, kb, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rbl, (lds_ptr)&sb[st][1][q * 16 * KB], 16, vo, kb, 0, 0);
+    }
+  };
+  floatx16 acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.0f;
 
-  const int fr = lane & 31, fk = 8 * (lane >> 5);
-  for (int k0 = 0; k0 < kp; k0 += KB) {
+  // fragment reads: lane (fr, fk) reads row fr of a 32-row tile, global
+  // chunk g = kk / 8 + fk / 8, stored at g ^ ((fr >> 1) & 3)
+  const int fr = lane & 31, fk = lane >> 5, sw = (fr >> 1) & 3;
+  stage(0, 0);
+  for (int k0 = 0, st = 0; k0 < kp; k0 += KB, st ^= 1) {
+    // this step's copies have landed (every wave's, after the barrier), and
+    // every wave is done with the other buffer's products
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    uint4 *dah = reinterpret_cast<uint4 *>(&sa[0][lr * LDK + lc]);
-    uint4 *dal = reinterpret_cast<uint4 *>(&sa[1][lr * LDK + lc]);
-    uint4 *dbh = reinterpret_cast<uint4 *>(&sb[0][lr * LDK + lc]);
-    uint4 *dbl = reinterpret_cast<uint4 *>(&sb[1][lr * LDK + lc]);
-    dah[0] = rah0; dah[1] = rah1; dal[0] = ral0; dal[1] = ral1;
-    dbh[0] = rbh0; dbh[1] = rbh1; dbl[0] = rbl0; dbl[1] = rbl1;
-    __syncthreads();
-    if (k0 + KB < kp) {
-      const int o = (k0 + KB) / 8;  // uint4 = 8 halves
-      rah0 = gah[o]; rah1 = gah[o + 1]; ral0 = gal[o]; ral1 = gal[o + 1];
-      rbh0 = gbh[o]; rbh1 = gbh[o + 1]; rbl0 = gbl[o]; rbl1 = gbl[o + 1];
-    }
+    if (k0 + KB < kp) stage(k0 + KB, st ^ 1);
 #pragma unroll
-    for (int kk = 0; kk < KB; kk += 16) {
-      half8 ah[2], al[2], bh[2], bl[2];
+    for (int kk = 0; kk < 2; ++kk) {
+      const int ch = ((2 * kk + fk) ^ sw) * 8;   // halves
+      half8 ah[4], al[4], bh[2], bl[2];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int ar = (wm * 64 + t * 32 + fr) * LDK + kk + fk;
-        const int br = (wn * 64 + t * 32 + fr) * LDK + kk + fk;
-        ah[t] = *reinterpret_cast<const half8 *>(&sa[0][ar]);
-        al[t] = *reinterpret_cast<const half8 *>(&sa[1][ar]);
-        bh[t] = *reinterpret_cast<const half8 *>(&sb[0][br]);
-        bl[t] = *reinterpret_cast<const half8 *>(&sb[1][br]);
+      for (int t = 0; t < 4; ++t) {
+        const int ar = (wm * 128 + t * 32 + fr) * KB + ch;
+        ah[t] = *reinterpret_cast<const half8 *>(&sa[st][0][ar]);
+        al[t] = *reinterpret_cast<const half8 *>(&sa[st][1][ar]);
       }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int t = 0; t < 2; ++t) {
+        const int br = (wn * 64 + t * 32 + fr) * KB + ch;
+        bh[t] = *reinterpret_cast<const half8 *>(&sb[st][0][br]);
+        bl[t] = *reinterpret_cast<const half8 *>(&sb[st][1][br]);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
@@ -310,13 +385,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
   }
   // D layout: lane holds column n = lane % 32, rows 8(i/4) + 4(lane/32) + i%4
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int n = n0 + wn * 64 + b * 32 + fr;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        const size_t m = m0 + wm * 64 + a * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+        const size_t m = m0 + wm * 128 + a * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
         if (m < nrows && n < nu) Y[m * nu + n] = acc[a][b][i] * (rinv[m] * inv_st);
       }
     }
@@ -382,11 +457,13 @@ int run_toeplitz_mfma(cbct_ctx ctx, int nu, int nv, size_t nrows, const float *i
   _Float16 *xh = reinterpret_cast<_Float16 *>(ws);
   _Float16 *xl = reinterpret_cast<_Float16 *>(ws + part);
   float *rinv = reinterpret_cast<float *>(ws + 2 * part);
+  const int vec = nu % 4 == 0 && nu <= SPLIT_V * 256 && reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                  reinterpret_cast<uintptr_t>(cosw) % 16 == 0;
   hipLaunchKernelGGL(k_split_rows, dim3((unsigned)(mp / 4)), dim3(256), 0, s, in, cosw, nu, nv,
-                     nrows, mp, (int)kp, angles, parker, delta, tan_delta, xh, xl, rinv);
+                     nrows, mp, (int)kp, angles, parker, delta, tan_delta, vec, xh, xl, rinv);
   CBCT_LAUNCH_CHECK(ctx, "k_split_rows");
   cbct_prof_mark(ctx, PROF_FILTER, 0, s);
-  hipLaunchKernelGGL(k_toeplitz_mfma, dim3((unsigned)nblocks), dim3(256), 0, s, xh, xl, rinv,
+  hipLaunchKernelGGL(k_toeplitz_mfma, dim3((unsigned)nblocks), dim3(512), 0, s, xh, xl, rinv,
                      reinterpret_cast<const _Float16 *>(ctx->d_thi),
                      reinterpret_cast<const _Float16 *>(ctx->d_tlo), out, nrows, nu, (int)kp, ntn,
                      (unsigned)nblocks, ctx->toe_inv_scale);
