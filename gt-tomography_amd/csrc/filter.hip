@@ -356,32 +356,37 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
     // every wave is done with the other buffer's products
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (k0 + KB < kp) stage(k0 + KB, st ^ 1);
+    // all of this step's fragments first: the compiler orders LDS reads
+    // after the next step's copies into the other buffer behind a wait for
+    // those copies, reads before them need none
+    half8 ah[2][4], al[2][4], bh[2][2], bl[2][2];
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
       const int ch = ((2 * kk + fk) ^ sw) * 8;   // halves
-      half8 ah[4], al[4], bh[2], bl[2];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int ar = (wm * 128 + t * 32 + fr) * KB + ch;
-        ah[t] = *reinterpret_cast<const half8 *>(&sa[st][0][ar]);
-        al[t] = *reinterpret_cast<const half8 *>(&sa[st][1][ar]);
+        ah[kk][t] = *reinterpret_cast<const half8 *>(&sa[st][0][ar]);
+        al[kk][t] = *reinterpret_cast<const half8 *>(&sa[st][1][ar]);
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const int br = (wn * 64 + t * 32 + fr) * KB + ch;
-        bh[t] = *reinterpret_cast<const half8 *>(&sb[st][0][br]);
-        bl[t] = *reinterpret_cast<const half8 *>(&sb[st][1][br]);
+        bh[kk][t] = *reinterpret_cast<const half8 *>(&sb[st][0][br]);
+        bl[kk][t] = *reinterpret_cast<const half8 *>(&sb[st][1][br]);
       }
+    }
+    if (k0 + KB < kp) stage(k0 + KB, st ^ 1);
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[kk][a], bh[kk][b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[kk][a], bl[kk][b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[kk][a], bh[kk][b], acc[a][b], 0, 0, 0);
         }
-    }
   }
   // D layout: lane holds column n = lane % 32, rows 8(i/4) + 4(lane/32) + i%4
 #pragma unroll
