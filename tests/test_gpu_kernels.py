@@ -368,7 +368,7 @@ FILTER_ENGINES = [0, 1]   # 0: FP32 VALU Toeplitz, 1: FP16 hi/lo split MFMA
 
 
 @pytest.mark.parametrize("engine", FILTER_ENGINES)
-@pytest.mark.parametrize("nu", [48, 37, 256])
+@pytest.mark.parametrize("nu", [48, 37, 256, 2052])
 @pytest.mark.parametrize("which", ["ramlak", "guo"])
 def test_ramp_filter_parity(ctx, nu, which, engine):
     print_case("filter", nu, which, "engine", engine)
@@ -382,6 +382,30 @@ def test_ramp_filter_parity(ctx, nu, which, engine):
     err = rel_l2(host(out), ref)
     print("filter rel-L2", nu, which, "engine", engine, err)
     assert err <= TOL
+
+
+@pytest.mark.parametrize("nu", [256, 1024])
+def test_mfma_filter_row_paths_bitwise(ctx, nu):
+    """The MFMA engine's row split keeps rows of up to 2048 elements in
+    registers when the input is 16-byte aligned and reads them twice
+    otherwise: a 4-byte-offset copy of the same rows must give the same
+    bits."""
+    print_case("filter row paths", nu)
+    rows = np.random.default_rng(nu + 1).standard_normal((70, nu)).astype(np.float32)
+    ctx.set_filter(O.ramlak_filter(nu))
+    ctx.filter_engine(1)
+    base = torch.empty(70 * nu + 4, device="cuda")
+    aligned = base[4:].view(70, nu)
+    shifted = base[1:70 * nu + 1].view(70, nu)
+    assert aligned.data_ptr() % 16 == 0 and shifted.data_ptr() % 16 == 4
+    aligned.copy_(torch.from_numpy(rows))
+    out_a = torch.empty((70, nu), device="cuda")
+    ctx.ramp_filter(nu, aligned, out_a)
+    shifted.copy_(torch.from_numpy(rows))
+    out_s = torch.empty((70, nu), device="cuda")
+    ctx.ramp_filter(nu, shifted, out_s)
+    torch.cuda.synchronize()
+    assert torch.equal(out_a, out_s)
 
 
 @pytest.mark.parametrize("engine", FILTER_ENGINES)
