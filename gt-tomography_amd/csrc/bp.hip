@@ -187,7 +187,8 @@ constexpr int BPP_CAP = 13 * 256;   // 48 rows of 68 elements, rounded up to who
 constexpr int BPP_CHUNKS = BPP_CAP / 256;
 
 // The rectangle of block L (x fastest, z slowest, as k_bpp) at view k, as
-// entry L * nviews + k: (u0 | v0 << 16, H), H = 0 when it does not fit.
+// entry L * nviews + k: (u0 | v0 << 16, H | inner << 8), 0 when it does not
+// fit.
 // Its bounds: the projections of the block's four corner columns at its
 // first and last z bound every voxel of the block (u is a linear-fractional
 // function of (x, y), v is linear in the magnification, so both are extreme
@@ -208,6 +209,7 @@ __global__ void __launch_bounds__(256) k_bpp_rects(BPParams P, const ViewF *__re
   const float pz0 = ((z0 + 0.5f) / P.nz - 0.5f) * P.lz;
   const ViewF vw = views[k];
   int ulo = 1 << 30, uhi = -(1 << 30), vlo = 1 << 30, vhi = -(1 << 30);
+  float rlo = 3.0e38f, rhi = -3.0e38f;   // unclamped row coordinates
   for (int c = 0; c < 4; ++c) {
     const int cxi = xb + ((c & 1) ? BP_TX - 1 : 0), cyi = yb + ((c & 2) ? BP_TY - 1 : 0);
     const float cpx = ((cxi + 0.5f) / P.nx - 0.5f) * P.lx;
@@ -217,8 +219,11 @@ __global__ void __launch_bounds__(256) k_bpp_rects(BPParams P, const ViewF *__re
     const float cmag = P.sdd * __builtin_amdgcn_rcpf(cqy + P.sad);
     const float ctu = __builtin_amdgcn_fmed3f(fmaf(fmaf(cqx, cmag, -vw.off_u), P.su, P.cu), 0.0f, P.clu);
     const float ctv0 = fmaf(fmaf(pz0, cmag, -vw.off_v), P.sv, P.cv);
+    const float ctv1 = fmaf((float)(KZ - 1), dz * cmag * P.sv, ctv0);
+    rlo = fminf(rlo, fminf(ctv0, ctv1));
+    rhi = fmaxf(rhi, fmaxf(ctv0, ctv1));
     const float ca = __builtin_amdgcn_fmed3f(ctv0, 0.0f, P.clv);
-    const float cb = __builtin_amdgcn_fmed3f(fmaf((float)(KZ - 1), dz * cmag * P.sv, ctv0), 0.0f, P.clv);
+    const float cb = __builtin_amdgcn_fmed3f(ctv1, 0.0f, P.clv);
     ulo = min(ulo, (int)ctu);
     uhi = max(uhi, (int)ctu);
     vlo = min(vlo, (int)fminf(ca, cb));
@@ -228,7 +233,10 @@ __global__ void __launch_bounds__(256) k_bpp_rects(BPParams P, const ViewF *__re
   const int W = min(uhi + 2, (int)P.pnu2 - 1) - u0 + 1;
   const int H = min(vhi + 2, (int)P.pnv - 1) - v0 + 1;
   const bool staged = W <= BPP_RW && H * BPP_RW <= BPP_CAP;
-  rects[t] = make_uint2((unsigned)u0 | ((unsigned)v0 << 16), staged ? (unsigned)H : 0u);
+  // bit 8: every voxel's row coordinate lies inside [0, nv + 1] (half a
+  // texel of margin), so its clamp is the identity and is skipped
+  const unsigned inner = rlo >= 0.5f && rhi <= P.clv - 0.5f ? 0x100u : 0u;
+  rects[t] = make_uint2((unsigned)u0 | ((unsigned)v0 << 16), staged ? (unsigned)H | inner : 0u);
 }
 
 template <bool PAIRS, bool FBP, int KZ>
@@ -273,7 +281,7 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bpp(BPParams P, const ViewF *_
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)((const char *)(copy + (size_t)k * P.pview2) + org), (short)0, (int)(view_bytes - org),
         0x00020000);
-    const int n = ((int)R.y * BPP_RW + 255) >> 8;
+    const int n = ((int)(R.y & 0xffu) * BPP_RW + 255) >> 8;
 #pragma unroll
     for (int j = 0; j < (BPP_CHUNKS + 3) / 4; ++j)
       if (wv + 4 * j < n)
@@ -335,11 +343,21 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bpp(BPParams P, const ViewF *_
       const unsigned colb = (unsigned)iu * 4u + s;
       const char *lds = (const char *)buf;
       auto ld = [&](unsigned o) { return *(const unsigned *)(lds + o); };
+      if (cur.y & 0x100u) {
+        // the row clamp is the identity for the whole block (same values)
 #pragma unroll
-      for (int t = 0; t < KZ; ++t) {
-        const float tv = __builtin_amdgcn_fmed3f(fmaf((float)t, dtv, tv0), 0.0f, P.clv);
-        taps(t, __builtin_amdgcn_fractf(tv), ld, __umul24((unsigned)(int)tv, P.bpp_rowb) + colb,
-             BPP_RW * 4u);
+        for (int t = 0; t < KZ; ++t) {
+          const float tv = fmaf((float)t, dtv, tv0);
+          taps(t, __builtin_amdgcn_fractf(tv), ld, __umul24((unsigned)(int)tv, P.bpp_rowb) + colb,
+               BPP_RW * 4u);
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < KZ; ++t) {
+          const float tv = __builtin_amdgcn_fmed3f(fmaf((float)t, dtv, tv0), 0.0f, P.clv);
+          taps(t, __builtin_amdgcn_fractf(tv), ld, __umul24((unsigned)(int)tv, P.bpp_rowb) + colb,
+               BPP_RW * 4u);
+        }
       }
     } else {
       const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
