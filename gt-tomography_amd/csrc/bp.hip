@@ -174,9 +174,9 @@ __global__ void __launch_bounds__(BP_TX *BP_TY) k_bp(BPParams P, const ViewF *__
 // (k_bpp_rects) is copied into LDS by 16-byte direct global-to-LDS loads, one
 // view ahead, and every voxel takes its taps from there.  Two layouts, both
 // 4-byte elements in rows of pnu2 (16-byte aligned): PAIRS (BF16 working
-// copies, k_pad_pair): element (a, b) holds the taps (a, b) and (a+1, b), so
+// copies, k_pad_staged): element (a, b) holds the taps (a, b) and (a+1, b), so
 // the four taps of a voxel-view are two elements one LDS row apart, one
-// ds_read2_b32; plain FP32 (k_pad_plain): the four taps are two adjacent
+// ds_read2_b32; plain FP32 (k_pad_staged): the four taps are two adjacent
 // pairs of elements one row apart, two ds_read2_b32.  LDS rows are BPP_RW =
 // 68 elements (adjacent rows 4 banks apart); a buffer holds BPP_CAP
 // elements, a multiple of one 64-lane copy (256), so whole copies never write
@@ -480,52 +480,59 @@ inline dim3 pad_grid(unsigned pnu, int nviews) {
   return dim3((pnu + 63) / 64, 16, (unsigned)std::min(nviews, 1024));
 }
 
-// Plain zero-bordered copy with the storage conversion (FP32 -> BF16).
-// Grid as k_pad_quad.
-template <typename TI, typename T>
-__global__ void __launch_bounds__(256) k_pad_plain(const TI *__restrict__ in, T *__restrict__ out,
-                                                   int nu, int nv, unsigned pnu, unsigned pnv,
-                                                   int nviews) {
-  const int a = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
-  if (a >= (int)pnu) return;
+// The staged backprojector's copies, rows of pnu2 (a multiple of 4) 4-byte
+// elements; element (a, b) of a view is, with u = a-1, v = b-1 and zero
+// outside the projection: PAIRS, the BF16 taps (u, v) (low half) and
+// (u+1, v); else the FP32 value at (u, v).  Each thread writes 4 elements
+// (one 16-byte store) from the projection values at u = a0-1 .. a0+3: one
+// 16-byte (FP32) or 8-byte (BF16) load plus the value before when the group
+// lies inside the row and vec (rows aligned for it) is set, else value by
+// value.  Grid as k_pad_quad over pnu2 / 4 columns.
+template <typename TI, bool PAIRS>
+__global__ void __launch_bounds__(256) k_pad_staged(const TI *__restrict__ in, unsigned *__restrict__ out,
+                                                    int nu, int nv, unsigned pnu2, unsigned pnv,
+                                                    int nviews, int vec) {
+  const int a0 = 4 * (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+  if (a0 >= (int)pnu2) return;
   const int b0 = (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), bs = (int)(gridDim.y * 4);
-  const int u = a - 1;
-  for (int k = (int)blockIdx.z; k < nviews; k += (int)gridDim.z) {
-    const TI *pk = in + (size_t)nu * nv * k;
-    T *ok = out + (size_t)pnu * pnv * k;
-    for (int b = b0; b < (int)pnv; b += bs) {
-      const int v = b - 1;
-      T val = T(0);
-      if (u >= 0 && u < nu && v >= 0 && v < nv) val = to_storage<T>(pk[(size_t)u + (size_t)nu * v]);
-      ok[(size_t)pnu * b + a] = val;
-    }
-  }
-}
-
-// Pair copy for k_bpp: dword (a, b) of a view holds the BF16 taps (a, b) (low
-// half) and (a+1, b) of the zero-bordered copy, i.e. the projection at
-// (u, v) = (a-1, b-1) and (a, b-1), zero outside; rows of pnu2 dwords.  Grid
-// as k_pad_quad over pnu2 columns.
-template <typename TI>
-__global__ void __launch_bounds__(256) k_pad_pair(const TI *__restrict__ in, unsigned *__restrict__ out,
-                                                  int nu, int nv, unsigned pnu2, unsigned pnv,
-                                                  int nviews) {
-  const int a = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
-  if (a >= (int)pnu2) return;
-  const int b0 = (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), bs = (int)(gridDim.y * 4);
-  const int u = a - 1;
+  const bool inner = vec && a0 >= 4 && a0 + 3 < nu;
   for (int k = (int)blockIdx.z; k < nviews; k += (int)gridDim.z) {
     const TI *pk = in + (size_t)nu * nv * k;
     unsigned *ok = out + (size_t)pnu2 * pnv * k;
     for (int b = b0; b < (int)pnv; b += bs) {
       const int v = b - 1;
-      unsigned lo = 0, hi = 0;
+      uint4 o = make_uint4(0u, 0u, 0u, 0u);
       if (v >= 0 && v < nv) {
         const TI *row = pk + (size_t)nu * v;
-        if (u >= 0 && u < nu) lo = to_storage<unsigned short>(row[u]);
-        if (u + 1 < nu) hi = to_storage<unsigned short>(row[u + 1]);
+        TI e[5];
+        if (inner) {
+          e[0] = row[a0 - 1];
+          if constexpr (sizeof(TI) == 4) {
+            const float4 q = *reinterpret_cast<const float4 *>(row + a0);
+            e[1] = q.x; e[2] = q.y; e[3] = q.z; e[4] = q.w;
+          } else {
+            const uint2 q = *reinterpret_cast<const uint2 *>(row + a0);
+            e[1] = (TI)(q.x & 0xffffu); e[2] = (TI)(q.x >> 16);
+            e[3] = (TI)(q.y & 0xffffu); e[4] = (TI)(q.y >> 16);
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 5; ++i) {
+            const int u = a0 - 1 + i;
+            e[i] = (u >= 0 && u < nu) ? row[u] : TI(0);
+          }
+        }
+        if constexpr (PAIRS) {
+          unsigned h[5];
+#pragma unroll
+          for (int i = 0; i < 5; ++i) h[i] = to_storage<unsigned short>(e[i]);
+          o = make_uint4(h[0] | (h[1] << 16), h[1] | (h[2] << 16), h[2] | (h[3] << 16), h[3] | (h[4] << 16));
+        } else {
+          o = make_uint4(__float_as_uint(e[0]), __float_as_uint(e[1]), __float_as_uint(e[2]),
+                         __float_as_uint(e[3]));
+        }
       }
-      ok[(size_t)pnu2 * b + a] = lo | (hi << 16);
+      *reinterpret_cast<uint4 *>(ok + (size_t)pnu2 * b + a0) = o;
     }
   }
 }
@@ -723,13 +730,14 @@ int run_bp(cbct_ctx ctx, const cbct_geom *g, const BPParams &P, dim3 grid, int k
     if (rc2) return rc2;
     unsigned *cp = static_cast<unsigned *>(ctx->ws->proj_pad.ptr);
     uint2 *rects = static_cast<uint2 *>(ctx->ws->bp_rects.ptr);
+    const int vec = g->nu % 4 == 0 && reinterpret_cast<uintptr_t>(proj) % (4 * sizeof(TI)) == 0;
     if (sizeof(T) == 2)
-      hipLaunchKernelGGL((k_pad_pair<TI>), pad_grid(P.pnu2, nviews), dim3(256), 0, s, proj, cp, g->nu,
-                         g->nv, P.pnu2, P.pnv, nviews);
+      hipLaunchKernelGGL((k_pad_staged<TI, true>), pad_grid(P.pnu2 / 4, nviews), dim3(256), 0, s, proj, cp,
+                         g->nu, g->nv, P.pnu2, P.pnv, nviews, vec);
     else
-      hipLaunchKernelGGL((k_pad_plain<TI, float>), pad_grid(P.pnu2, nviews), dim3(256), 0, s, proj,
-                         reinterpret_cast<float *>(cp), g->nu, g->nv, P.pnu2, P.pnv, nviews);
-    CBCT_LAUNCH_CHECK(ctx, "k_pad_pair / k_pad_plain");
+      hipLaunchKernelGGL((k_pad_staged<TI, false>), pad_grid(P.pnu2 / 4, nviews), dim3(256), 0, s, proj,
+                         cp, g->nu, g->nv, P.pnu2, P.pnv, nviews, vec);
+    CBCT_LAUNCH_CHECK(ctx, "k_pad_staged");
     const ViewF *vf = ctx->d_views_f + view0;
     const size_t nr = nblocks * nviews;
     if (kz == 8)
