@@ -18,8 +18,8 @@
   and bounded), backprojection engines bitwise equal at each z-depth, and the
   split-precision MFMA ramp filter on the real 1024-wide rows.
 * 64 uneven scans (edges 96-320, 90-540 views, 10-iteration OS-SART): all
-  of them through the server, served == alone bitwise (FP32 and BF16
-  working copies); skipped, see the test.
+  of them through the server on one lane == alone bitwise (FP32 and BF16
+  working copies); the four-lane served run is skipped, see that test.
 * A Poisson stream of 60% FDK / 40% OS-SART requests at the same sizes:
   served == alone bitwise.
 """
@@ -438,6 +438,29 @@ def test_uneven64_served_equals_alone(uneven64, storage):
         res, wall = S.run_stream(srv, served)
     summary = S.summarize(served, res, wall)
     print("uneven64", storage, summary)
+    for i, (out, want) in enumerate(zip(res, alone)):
+        assert np.all(np.isfinite(want)), i
+        assert np.array_equal(out["volume"], want), i
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_uneven64_through_server_equals_alone(uneven64, storage):
+    """All 64 scans through the server on one lane (the request path the
+    lanes share, one request at a time) equal the same scans run alone."""
+    reqs = uneven64
+    print_case("uneven64 one-lane server vs alone", storage)
+    alone = []
+    ops.set_default_storage(storage)
+    try:
+        for r in reqs:
+            alone.append(ops.os_sart(r.geom, r.angles, None, r.projections, r.subsets,
+                                     r.iterations, beta=r.beta, gamma=r.gamma, nonneg=r.nonneg))
+    finally:
+        ops.set_default_storage("f32")
+    queued = [dataclasses.replace(r, volume=None, storage=storage) for r in reqs]
+    with S.Server(lanes=1) as srv:
+        res, wall = S.run_stream(srv, queued)
+    print("uneven64 one lane", storage, S.summarize(queued, res, wall), flush=True)
     for i, (out, want) in enumerate(zip(res, alone)):
         assert np.all(np.isfinite(want)), i
         assert np.array_equal(out["volume"], want), i
