@@ -78,9 +78,9 @@ struct cbct_ctx_s {
   int work_dtype = CBCT_F32;     // type of the projectors' working copies of FP32 arrays
   float *d_h = nullptr;          // fp32 spatial kernel, 2*nu entries (h[|m|] mirrored)
   std::vector<double> h64;       // the same kernel in FP64, h[0..nu-1]
-  unsigned short *d_thi = nullptr; // fp16 hi Toeplitz matrix [np][kp] (built on first use)
-  unsigned short *d_tlo = nullptr; // fp16 lo Toeplitz matrix
-  int toe_ld = 0;                // kp: leading dimension of the fp16 matrices
+  unsigned short *d_thi = nullptr; // fp16 hi Toeplitz row, 8 shifted copies (built on first use)
+  unsigned short *d_tlo = nullptr; // fp16 lo parts, the same layout
+  int toe_ld = 0;                // length of one copy (halves)
   float toe_inv_scale = 1.0f;    // inverse of the power-of-two scale applied to them
   // cosine weights cache
   float *d_cosw = nullptr;

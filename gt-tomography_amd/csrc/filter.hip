@@ -168,6 +168,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // Block tile MB x NB, K-steps of KB (one LDS row of a part = KB halves =
 // 64 bytes = four 16-byte chunks).
 constexpr int MB = 256, NB = 256, KB = 32;
+// (the MFMA kernel's fragment reuse across K-steps needs KB = the 32-wide MFMA tile)
+static_assert(KB == 32, "K-step = MFMA tile width");
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -286,10 +288,13 @@ __global__ void __launch_bounds__(256) k_split_rows(const float *__restrict__ in
 
 // 512 threads: eight waves in a 2 x 4 arrangement, each owning a 128 x 64
 // output block as 4 x 2 tiles of the 32x32x16 fp16 MFMA.  Per K-step the
-// block's 256 x 32 row slice and 256 x 32 Toeplitz slice (hi and lo parts)
-// go straight from global memory into LDS (16-byte direct loads, no
-// registers), double-buffered: the next step's copies are issued before this
-// step's products, one barrier per step.  LDS rows are 64 bytes; chunk c of
+// block's 256 x 32 row slice (hi and lo parts) goes straight from global
+// memory into LDS (16-byte direct loads, no registers), double-buffered: the
+// next step's copies are issued before this step's products, one barrier
+// per step.  The Toeplitz operand never goes through LDS: each fragment is 8
+// consecutive entries of its generating row (ensure_toeplitz_f16), loaded
+// into registers, and one K-step (32) along k equals one tile along n, so a
+// wave loads one new tile per step and shifts the other.  LDS rows are 64 bytes; chunk c of
 // row r is stored at chunk c ^ ((r >> 1) & 3), so the 8 rows a group of 8
 // lanes reads at one chunk fall in 8 different bank groups, and a direct load
 // (64 lanes -> 1 KB contiguous) still fills whole rows.  One block (two
@@ -299,13 +304,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
                                                        const _Float16 *__restrict__ xl,
                                                        const float *__restrict__ rinv,
                                                        const _Float16 *__restrict__ th,
-                                                       const _Float16 *__restrict__ tl,
+                                                       const _Float16 *__restrict__ tl, int tlen,
                                                        float *__restrict__ Y, size_t nrows, int nu,
                                                        int kp, unsigned ntn, unsigned nblocks,
                                                        float inv_st) {
-  // [stage][part][row][KB halves]: rows of the block's X slice, rows (n) of T's
+  // [stage][part][row][KB halves]: rows of the block's X slice
   __shared__ __attribute__((aligned(1024))) _Float16 sa[2][2][MB * KB];
-  __shared__ __attribute__((aligned(1024))) _Float16 sb[2][2][NB * KB];
   const unsigned L = xcd_remap(blockIdx.x, nblocks);
   const unsigned tn = L % ntn, tm = L / ntn;
   const size_t m0 = (size_t)tm * MB;
@@ -323,8 +327,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
     return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
   };
   const __amdgpu_buffer_rsrc_t rah = rsrc(xh + m0 * kp, MB * rowb), ral = rsrc(xl + m0 * kp, MB * rowb);
-  const __amdgpu_buffer_rsrc_t rbh = rsrc(th + (size_t)n0 * kp, NB * rowb),
-                               rbl = rsrc(tl + (size_t)n0 * kp, NB * rowb);
+  const __amdgpu_buffer_rsrc_t rbh = rsrc(th, 8 * (size_t)tlen * sizeof(_Float16)),
+                               rbl = rsrc(tl, 8 * (size_t)tlen * sizeof(_Float16));
   typedef __attribute__((address_space(3))) void *lds_ptr;
   // wave w copies 16-row groups w and w + 8 of each part of both slices
   auto stage = [&](int k0, int st) {
@@ -335,8 +339,6 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
       const int vo = coff + q * 16 * (int)rowb;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(rah, (lds_ptr)&sa[st][0][q * 16 * KB], 16, vo, kb, 0, 0);
       __builtin_amdgcn_raw_ptr_buffer_load_lds(ral, (lds_ptr)&sa[st][1][q * 16 * KB], 16, vo, kb, 0, 0);
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rbh, (lds_ptr)&sb[st][0][q * 16 * KB], 16, vo, kb, 0, 0);
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rbl, (lds_ptr)&sb[st][1][q * 16 * KB], 16, vo, kb, 0, 0);
     }
   };
   floatx16 acc[4][2];
@@ -350,6 +352,32 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
   // fragment reads: lane (fr, fk) reads row fr of a 32-row tile, global
   // chunk g = kk / 8 + fk / 8, stored at g ^ ((fr >> 1) & 3)
   const int fr = lane & 31, fk = lane >> 5, sw = (fr >> 1) & 3;
+  // T's fragments come from the phase copies of its generating row (see
+  // ensure_toeplitz_f16): lane (fr, fk) of tile t needs T[n][k .. k+7] =
+  // G[S0 - n + k ..] with k = k0 + 16 kk + 8 fk, 16-byte aligned in copy
+  // q = (S0 - n) & 7 (k is a multiple of 8).  Byte offsets at k0 = 0:
+  const int S0 = (int)ntn * NB - 1;
+  unsigned boff[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int i0 = S0 - (n0 + wn * 64 + t * 32 + fr);
+    const int q = i0 & 7;
+    boff[t] = (unsigned)(q * tlen + (i0 - q) + 8 * fk) * (unsigned)sizeof(_Float16);
+  }
+  auto load_b = [&](int k0, int t, int kk) __attribute__((always_inline)) {
+    const int so = k0 * (int)sizeof(_Float16);
+    return __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(
+                                         t < 0 ? rbl : rbh, (int)boff[t < 0 ? -1 - t : t] + 32 * kk, so, 0));
+  };
+  // hi parts as t = 0, 1; lo parts as t = -1, -2
+  half8 bh[2][2], bl[2][2];
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bh[kk][t] = load_b(0, t, kk);
+      bl[kk][t] = load_b(0, -1 - t, kk);
+    }
   stage(0, 0);
   for (int k0 = 0, st = 0; k0 < kp; k0 += KB, st ^= 1) {
     // this step's copies have landed (every wave's, after the barrier), and
@@ -359,7 +387,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
     // all of this step's fragments first: the compiler orders LDS reads
     // after the next step's copies into the other buffer behind a wait for
     // those copies, reads before them need none
-    half8 ah[2][4], al[2][4], bh[2][2], bl[2][2];
+    half8 ah[2][4], al[2][4];
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
       const int ch = ((2 * kk + fk) ^ sw) * 8;   // halves
@@ -369,14 +397,20 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
         ah[kk][t] = *reinterpret_cast<const half8 *>(&sa[st][0][ar]);
         al[kk][t] = *reinterpret_cast<const half8 *>(&sa[st][1][ar]);
       }
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int br = (wn * 64 + t * 32 + fr) * KB + ch;
-        bh[kk][t] = *reinterpret_cast<const half8 *>(&sb[st][0][br]);
-        bl[kk][t] = *reinterpret_cast<const half8 *>(&sb[st][1][br]);
-      }
     }
     if (k0 + KB < kp) stage(k0 + KB, st ^ 1);
+    // this step's T fragments are in registers.  T is Toeplitz and KB is
+    // the tile width, so the next step's tile 1 is this step's tile 0
+    // (T[n + 32][k + 32] = T[n][k]); only its tile 0 is loaded, now, so it
+    // arrives during this step's products
+    half8 bhn[2], bln[2];
+    if (k0 + KB < kp) {
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        bhn[kk] = load_b(k0 + KB, 0, kk);
+        bln[kk] = load_b(k0 + KB, -1, kk);
+      }
+    }
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
@@ -387,6 +421,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[kk][a], bl[kk][b], acc[a][b], 0, 0, 0);
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[kk][a], bh[kk][b], acc[a][b], 0, 0, 0);
         }
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      bh[kk][1] = bh[kk][0];
+      bl[kk][1] = bl[kk][0];
+      bh[kk][0] = bhn[kk];
+      bl[kk][0] = bln[kk];
+    }
   }
   // D layout: lane holds column n = lane % 32, rows 8(i/4) + 4(lane/32) + i%4
 #pragma unroll
@@ -422,13 +463,24 @@ int ensure_toeplitz_f16(cbct_ctx ctx) {
   int ex = 0;
   if (hmax > 0.0) (void)std::frexp(hmax, &ex);
   const double sc = std::ldexp(1.0, 14 - ex);
-  std::vector<_Float16> hi(np * kp, (_Float16)0.0f), lo(np * kp, (_Float16)0.0f);
-  for (size_t n = 0; n < (size_t)nu; ++n)
-    for (size_t k = 0; k < (size_t)nu; ++k) {
-      const double v = ctx->h64[n > k ? n - k : k - n] * sc;
+  // T[n][k] = h[|n - k|] for n, k < nu (scaled, hi + lo fp16).  Stored as
+  // its generating row read backwards, G[s] = g[S0 - s] with g[t] = h[|t|]
+  // (0 for |t| >= nu), S0 = np - 1, so that T[n][k + j] = G[S0 - n + k + j]
+  // for j = 0..7: one MFMA fragment is 8 consecutive entries.  Eight copies,
+  // copy q holding G shifted by q (G_q[s] = G[s + q]), make every fragment a
+  // 16-byte aligned load.  Rows n >= nu and columns k >= nu of this T need
+  // not be zero: those outputs are discarded and those row entries are.
+  const long S0 = (long)np - 1;
+  const size_t tlen = round_up((size_t)S0 + kp, 8);
+  std::vector<_Float16> hi(8 * tlen), lo(8 * tlen);
+  for (size_t q = 0; q < 8; ++q)
+    for (size_t i = 0; i < tlen; ++i) {
+      const long t = S0 - (long)(i + q);
+      const long at = t < 0 ? -t : t;
+      const double v = at < nu ? ctx->h64[at] * sc : 0.0;
       const _Float16 h = (_Float16)v;
-      hi[n * kp + k] = h;
-      lo[n * kp + k] = (_Float16)(v - (double)h);
+      hi[q * tlen + i] = h;
+      lo[q * tlen + i] = (_Float16)(v - (double)h);
     }
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_thi, sizeof(_Float16) * hi.size()));
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_tlo, sizeof(_Float16) * lo.size()));
@@ -436,7 +488,7 @@ int ensure_toeplitz_f16(cbct_ctx ctx) {
   if (rc) return rc;
   rc = cbct_upload(ctx, ctx->d_tlo, lo.data(), sizeof(_Float16) * lo.size());
   if (rc) return rc;
-  ctx->toe_ld = (int)kp;
+  ctx->toe_ld = (int)tlen;
   ctx->toe_inv_scale = (float)std::ldexp(1.0, ex - 14);
   return CBCT_OK;
 }
@@ -449,7 +501,7 @@ int run_toeplitz_mfma(cbct_ctx ctx, int nu, int nv, size_t nrows, const float *i
   if (nrows == 0) return CBCT_OK;
   int rc = ensure_toeplitz_f16(ctx);
   if (rc) return rc;
-  const size_t kp = (size_t)ctx->toe_ld;
+  const size_t kp = round_up(nu, KB);
   const size_t mp = round_up(nrows, MB);
   const unsigned ntn = (unsigned)(round_up(nu, NB) / NB);
   const size_t nblocks = mp / MB * ntn;
@@ -470,8 +522,8 @@ int run_toeplitz_mfma(cbct_ctx ctx, int nu, int nv, size_t nrows, const float *i
   cbct_prof_mark(ctx, PROF_FILTER, 0, s);
   hipLaunchKernelGGL(k_toeplitz_mfma, dim3((unsigned)nblocks), dim3(512), 0, s, xh, xl, rinv,
                      reinterpret_cast<const _Float16 *>(ctx->d_thi),
-                     reinterpret_cast<const _Float16 *>(ctx->d_tlo), out, nrows, nu, (int)kp, ntn,
-                     (unsigned)nblocks, ctx->toe_inv_scale);
+                     reinterpret_cast<const _Float16 *>(ctx->d_tlo), ctx->toe_ld, out, nrows, nu,
+                     (int)kp, ntn, (unsigned)nblocks, ctx->toe_inv_scale);
   cbct_prof_mark(ctx, PROF_FILTER, 1, s);
   CBCT_LAUNCH_CHECK(ctx, "k_toeplitz_mfma");
   return CBCT_OK;
