@@ -25,6 +25,7 @@ from . import phantom as PH
 
 REQ_FDK = 0
 REQ_OS_SART = 1
+GTT_WAIT_TIMEOUT = 2
 
 
 class _Request(ctypes.Structure):
@@ -55,6 +56,8 @@ def _lib():
         L.gtt_server_destroy.argtypes = [P]
         L.gtt_server_submit.argtypes = [P, ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_longlong)]
         L.gtt_server_wait.argtypes = [P, ctypes.c_longlong, ctypes.POINTER(_Result)]
+        L.gtt_server_wait_for.argtypes = [P, ctypes.c_longlong, ctypes.c_longlong, ctypes.POINTER(_Result)]
+        L.gtt_server_wait_for.restype = I
         L.gtt_server_last_error.argtypes = [P]
         L.gtt_server_last_error.restype = ctypes.c_char_p
         L.gtt_server_set_autotune.argtypes = [P, I]
@@ -139,9 +142,17 @@ class Server:
         self._live[rid.value] = req      # keeps the buffers alive until wait()
         return rid.value
 
-    def wait(self, rid):
+    def wait(self, rid, timeout=None):
+        """Result of request rid; with timeout (seconds), TimeoutError when it
+        has not finished by then (it stays pending and may be waited for
+        again)."""
         r = _Result()
-        rc = _lib().gtt_server_wait(self.h, rid, ctypes.byref(r))
+        if timeout is None:
+            rc = _lib().gtt_server_wait(self.h, rid, ctypes.byref(r))
+        else:
+            rc = _lib().gtt_server_wait_for(self.h, rid, int(timeout * 1000), ctypes.byref(r))
+            if rc == GTT_WAIT_TIMEOUT:
+                raise TimeoutError(_lib().gtt_server_last_error(self.h).decode())
         req = self._live.pop(rid, None)
         if rc != 0:
             raise RuntimeError(r.message.decode() or _lib().gtt_server_last_error(self.h).decode())
@@ -220,9 +231,10 @@ def poisson_arrivals(count, rate_per_s, seed=7):
     return np.cumsum(rng.exponential(1.0 / rate_per_s, count))
 
 
-def run_stream(server: Server, requests, arrivals_s=None):
+def run_stream(server: Server, requests, arrivals_s=None, timeout=None):
     """Submit each request at its arrival time (all at once if None), wait
-    for all; returns per-request results and the wall time."""
+    for all (each wait at most `timeout` seconds when given); returns
+    per-request results and the wall time."""
     n = len(requests)
     arrivals_s = np.zeros(n) if arrivals_s is None else np.asarray(arrivals_s)
     t0 = time.perf_counter()
@@ -232,7 +244,7 @@ def run_stream(server: Server, requests, arrivals_s=None):
         if dt > 0:
             time.sleep(dt)
         ids.append(server.submit(req))
-    res = [server.wait(i) for i in ids]
+    res = [server.wait(i, timeout) for i in ids]
     wall = time.perf_counter() - t0
     return res, wall
 

@@ -319,16 +319,25 @@ int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id) {
   return 0;
 }
 
-int gtt_server_wait(gtt_server s, long long id, gtt_result *res) {
+namespace {
+// Wait for request id (without bound when timeout_ms < 0); GTT_WAIT_TIMEOUT
+// leaves it registered so it can be waited for again.
+int wait_request(gtt_server s, long long id, long long timeout_ms, gtt_result *res, const char *who) {
   if (!s) return -1;
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->jobs.find(id);
   if (it == s->jobs.end()) {
-    s->err = "gtt_server_wait: unknown request id " + std::to_string(id);
+    s->err = std::string(who) + ": unknown request id " + std::to_string(id);
     return -1;
   }
   std::shared_ptr<Job> j = it->second;
-  s->cv.wait(lk, [&] { return j->done; });
+  if (timeout_ms < 0) {
+    s->cv.wait(lk, [&] { return j->done; });
+  } else if (!s->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return j->done; })) {
+    s->err = std::string(who) + ": request " + std::to_string(id) + " not finished after " +
+             std::to_string(timeout_ms) + " ms";
+    return GTT_WAIT_TIMEOUT;
+  }
   s->jobs.erase(id);
   if (res) {
     res->status = j->status;
@@ -340,6 +349,19 @@ int gtt_server_wait(gtt_server s, long long id, gtt_result *res) {
   }
   if (j->status) s->err = j->message;
   return j->status;
+}
+}  // namespace
+
+int gtt_server_wait(gtt_server s, long long id, gtt_result *res) {
+  return wait_request(s, id, -1, res, "gtt_server_wait");
+}
+
+int gtt_server_wait_for(gtt_server s, long long id, long long timeout_ms, gtt_result *res) {
+  if (timeout_ms < 0) {
+    if (s) s->err = "gtt_server_wait_for: negative time limit";
+    return -1;
+  }
+  return wait_request(s, id, timeout_ms, res, "gtt_server_wait_for");
 }
 
 }  // extern "C"

@@ -249,6 +249,13 @@ int gtt_server_set_autotune(gtt_server s, int on);
 int gtt_server_set_graphs(gtt_server s, int on);
 int gtt_server_submit(gtt_server s, const gtt_request *req, long long *id);
 int gtt_server_wait(gtt_server s, long long id, gtt_result *res);
+/* gtt_server_wait with a time limit: GTT_WAIT_TIMEOUT (the request is still
+ * queued or running and may be waited for again; its buffers stay in use)
+ * when it has not finished within timeout_ms milliseconds, else as
+ * gtt_server_wait.  For callers that must not block without bound on a
+ * request that stops making progress.  No counterpart in the reference. */
+enum { GTT_WAIT_TIMEOUT = 2 };
+int gtt_server_wait_for(gtt_server s, long long id, long long timeout_ms, gtt_result *res);
 const char *gtt_server_last_error(gtt_server s);
 size_t gtt_request_footprint(const gtt_request *req);
 

@@ -78,6 +78,16 @@ def test_libgtt_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_server_wait_for_argument_errors():
+    """gtt_server_wait_for refuses a null server (host-only check; the time
+    limit itself is tested on the GPU)."""
+    from gtt import serving as S
+    L = S._lib()
+    assert L.gtt_server_wait_for(None, 1, 10, None) == -1
+    assert L.gtt_server_wait(None, 1, None) == -1
+    assert S.GTT_WAIT_TIMEOUT == 2
+
+
 def test_request_footprint_and_server_without_gpu():
     """Host-only parts of the server API: the footprint estimate (volume- and
     padded-projection-sized arrays, by request kind and subset count, plus

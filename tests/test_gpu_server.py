@@ -111,6 +111,24 @@ def test_failed_request_reports_and_server_continues(small_requests):
         assert np.all(np.isfinite(srv.wait(ig)["volume"]))
 
 
+def test_wait_with_time_limit(small_requests):
+    """gtt_server_wait_for: a request not finished within the limit reports
+    GTT_WAIT_TIMEOUT and stays pending; waited for again it returns the same
+    result as gtt_server_wait."""
+    req = next(r for r in small_requests if r.kind == S.REQ_OS_SART)
+    print_case("server wait with a time limit", req.geom.nx, len(req.angles))
+    ref = alone(req)
+    req.volume = None
+    with S.Server(lanes=1) as srv:
+        rid = srv.submit(req)
+        with pytest.raises(TimeoutError, match="not finished"):
+            srv.wait(rid, timeout=0.0)
+        out = srv.wait(rid, timeout=120.0)
+        assert np.array_equal(out["volume"], ref)
+        with pytest.raises(RuntimeError, match="unknown request"):
+            srv.wait(rid, timeout=1.0)
+
+
 def test_served_bf16_storage_equals_alone(small_requests):
     """A request with BF16 working copies, served beside FP32 requests,
     equals the same reconstruction run alone with BF16 working copies
