@@ -364,19 +364,18 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int q = i0 & 7;
     boff[t] = (unsigned)(q * tlen + (i0 - q) + 8 * fk) * (unsigned)sizeof(_Float16);
   }
-  auto load_b = [&](int k0, int t, int kk) __attribute__((always_inline)) {
-    const int so = k0 * (int)sizeof(_Float16);
+  // fragment (tile t, k-half kk) of part r (hi or lo) at K-step k0
+  auto load_b = [&](const __amdgpu_buffer_rsrc_t &r, int k0, int t, int kk) __attribute__((always_inline)) {
     return __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(
-                                         t < 0 ? rbl : rbh, (int)boff[t < 0 ? -1 - t : t] + 32 * kk, so, 0));
+                                         r, (int)boff[t] + 32 * kk, k0 * (int)sizeof(_Float16), 0));
   };
-  // hi parts as t = 0, 1; lo parts as t = -1, -2
   half8 bh[2][2], bl[2][2];
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      bh[kk][t] = load_b(0, t, kk);
-      bl[kk][t] = load_b(0, -1 - t, kk);
+      bh[kk][t] = load_b(rbh, 0, t, kk);
+      bl[kk][t] = load_b(rbl, 0, t, kk);
     }
   stage(0, 0);
   for (int k0 = 0, st = 0; k0 < kp; k0 += KB, st ^= 1) {
@@ -407,8 +406,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
     if (k0 + KB < kp) {
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk) {
-        bhn[kk] = load_b(k0 + KB, 0, kk);
-        bln[kk] = load_b(k0 + KB, -1, kk);
+        bhn[kk] = load_b(rbh, k0 + KB, 0, kk);
+        bln[kk] = load_b(rbl, k0 + KB, 0, kk);
       }
     }
 #pragma unroll
