@@ -158,18 +158,21 @@ __global__ void __launch_bounds__(256) k_toeplitz_f32(const float *__restrict__ 
 // fp16 hi + lo parts after a power-of-two scaling (per row for X, one scale
 // for T), and three products hi*hi + hi*lo + lo*hi accumulate in FP32; the
 // dropped lo*lo term is below 2^-22 of |x||t| per product.
-//   X parts: [mp][kp] halves (mp = rows rounded up to 128, kp = nu rounded up
-//            to 32, zero padded), row scale inverse rinv[mp].
-//   T parts: [np][kp] halves, np = nu rounded up to 128 (rows n, contiguous
-//            over k, which for a symmetric T is also the transpose).
+//   X parts: [mp][kp] halves (mp = rows rounded up to MB, kp = nu rounded up
+//            to KB, zero padded), row scale inverse rinv[mp].
+//   T parts: 8 shifted copies of T's generating row (ensure_toeplitz_f16),
+//            np = nu rounded up to NB; T is symmetric, so rows n over k are
+//            also its transpose.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // Block tile MB x NB, K-steps of KB (one LDS row of a part = KB halves =
-// 64 bytes = four 16-byte chunks).
-constexpr int MB = 256, NB = 256, KB = 32;
-// (the MFMA kernel's fragment reuse across K-steps needs KB = the 32-wide MFMA tile)
-static_assert(KB == 32, "K-step = MFMA tile width");
+// 128 bytes = eight 16-byte chunks).
+constexpr int MB = 256, NB = 256, KB = 64;
+// the Toeplitz operand advances in K-chunks of one 32-wide MFMA tile, two per
+// LDS K-step (its fragment reuse needs chunk = tile width)
+constexpr int KC = 32;
+static_assert(KB == 2 * KC, "two Toeplitz chunks per K-step");
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -255,7 +258,7 @@ __global__ void __launch_bounds__(256) k_split_rows(const float *__restrict__ in
 #pragma unroll
     for (int i = 0; i < SPLIT_V; ++i) {
       const int u = 4 * lane + 256 * i;
-      if (u < kp) {   // kp - nu < 32 and both are multiples of 4: whole groups
+      if (u < kp) {   // kp - nu < 64 and both are multiples of 4: whole groups
         _Float16 h[4], l[4];
         split(v[i].x * sc, h[0], l[0]);
         split(v[i].y * sc, h[1], l[1]);
@@ -288,16 +291,17 @@ __global__ void __launch_bounds__(256) k_split_rows(const float *__restrict__ in
 
 // 512 threads: eight waves in a 2 x 4 arrangement, each owning a 128 x 64
 // output block as 4 x 2 tiles of the 32x32x16 fp16 MFMA.  Per K-step the
-// block's 256 x 32 row slice (hi and lo parts) goes straight from global
+// block's 256 x 64 row slice (hi and lo parts) goes straight from global
 // memory into LDS (16-byte direct loads, no registers), double-buffered: the
 // next step's copies are issued before this step's products, one barrier
 // per step.  The Toeplitz operand never goes through LDS: each fragment is 8
 // consecutive entries of its generating row (ensure_toeplitz_f16), loaded
-// into registers, and one K-step (32) along k equals one tile along n, so a
-// wave loads one new tile per step and shifts the other.  LDS rows are 64 bytes; chunk c of
-// row r is stored at chunk c ^ ((r >> 1) & 3), so the 8 rows a group of 8
-// lanes reads at one chunk fall in 8 different bank groups, and a direct load
-// (64 lanes -> 1 KB contiguous) still fills whole rows.  One block (two
+// into registers, and one K-chunk (32) along k equals one tile along n, so a
+// wave loads one new tile per chunk (two chunks per K-step) and shifts the
+// other.  LDS rows are 128 bytes; chunk c of row r is stored at chunk
+// c ^ (r & 7), so the 8 rows a group of 8 lanes reads at one chunk fall in 8
+// different bank groups, and a direct load (64 lanes -> 1 KB contiguous)
+// fills 8 whole rows.  One block (two
 // waves per SIMD) per CU.  The products accumulate in the order of K, hi*hi,
 // hi*lo, lo*hi per 16 of K.
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) k_toeplitz_mfma(const _Float16 *__restrict__ xh,
@@ -317,12 +321,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 2, wn = wave & 3;
-  // direct loads: a 1 KB copy fills 16 rows; lane -> row lane / 4 of them,
-  // stored chunk lane & 3, holding global chunk (lane & 3) ^ ((row >> 1) & 3)
-  // (16-row groups start at even rows, so (row >> 1) & 3 depends on the lane only)
-  const int crow = lane >> 2, cchunk = (lane & 3) ^ ((crow >> 1) & 3);
+  // direct loads: a 1 KB copy fills 8 rows of 128 bytes; lane -> row lane / 8
+  // of them, stored chunk lane & 7, holding global chunk (lane & 7) ^ (row & 7)
+  // (8-row groups start at multiples of 8, so row & 7 = lane / 8)
+  const int crow = lane >> 3, cchunk = (lane & 7) ^ crow;
   const size_t rowb = (size_t)kp * sizeof(_Float16);
-  const int coff = crow * (int)rowb + cchunk * 16;   // bytes, + 16 rows per copy
+  const int coff = crow * (int)rowb + cchunk * 16;   // bytes, + 8 rows per copy
   auto rsrc = [](const _Float16 *p, size_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
   };
@@ -330,15 +334,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
   const __amdgpu_buffer_rsrc_t rbh = rsrc(th, 8 * (size_t)tlen * sizeof(_Float16)),
                                rbl = rsrc(tl, 8 * (size_t)tlen * sizeof(_Float16));
   typedef __attribute__((address_space(3))) void *lds_ptr;
-  // wave w copies 16-row groups w and w + 8 of each part of both slices
+  // wave w copies 8-row groups w, w + 8, w + 16 and w + 24 of both parts
+  // (the group's row offset is uniform: scalar offset; the lane's, one VGPR)
   auto stage = [&](int k0, int st) {
-    const int kb = k0 * (int)sizeof(_Float16);
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < 4; ++j) {
       const int q = wave + 8 * j;
-      const int vo = coff + q * 16 * (int)rowb;
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rah, (lds_ptr)&sa[st][0][q * 16 * KB], 16, vo, kb, 0, 0);
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(ral, (lds_ptr)&sa[st][1][q * 16 * KB], 16, vo, kb, 0, 0);
+      const int so = k0 * (int)sizeof(_Float16) + q * 8 * (int)rowb;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rah, (lds_ptr)&sa[st][0][q * 8 * KB], 16, coff, so, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(ral, (lds_ptr)&sa[st][1][q * 8 * KB], 16, coff, so, 0, 0);
     }
   };
   floatx16 acc[4][2];
@@ -350,8 +354,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
       for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.0f;
 
   // fragment reads: lane (fr, fk) reads row fr of a 32-row tile, global
-  // chunk g = kk / 8 + fk / 8, stored at g ^ ((fr >> 1) & 3)
-  const int fr = lane & 31, fk = lane >> 5, sw = (fr >> 1) & 3;
+  // chunk g = 2 kk + fk (8 halves each), stored at g ^ (fr & 7)
+  const int fr = lane & 31, fk = lane >> 5, sw = fr & 7;
   // T's fragments come from the phase copies of its generating row (see
   // ensure_toeplitz_f16): lane (fr, fk) of tile t needs T[n][k .. k+7] =
   // G[S0 - n + k ..] with k = k0 + 16 kk + 8 fk, 16-byte aligned in copy
@@ -364,7 +368,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int q = i0 & 7;
     boff[t] = (unsigned)(q * tlen + (i0 - q) + 8 * fk) * (unsigned)sizeof(_Float16);
   }
-  // fragment (tile t, k-half kk) of part r (hi or lo) at K-step k0
+  // fragment (tile t, k-half kk) of part r (hi or lo) at K-chunk k0
   auto load_b = [&](const __amdgpu_buffer_rsrc_t &r, int k0, int t, int kk) __attribute__((always_inline)) {
     return __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(
                                          r, (int)boff[t] + 32 * kk, k0 * (int)sizeof(_Float16), 0));
@@ -383,49 +387,53 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))
     // every wave is done with the other buffer's products
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // all of this step's fragments first: the compiler orders LDS reads
-    // after the next step's copies into the other buffer behind a wait for
-    // those copies, reads before them need none
-    half8 ah[2][4], al[2][4];
 #pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
-      const int ch = ((2 * kk + fk) ^ sw) * 8;   // halves
+    for (int h = 0; h < 2; ++h) {
+      const int kc = k0 + h * KC;   // this half's 32 of K
+      // the half's T fragments are in registers.  T is Toeplitz and KC is
+      // the tile width, so the next chunk's tile 1 is this chunk's tile 0
+      // (T[n + 32][k + 32] = T[n][k]); only its tile 0 is loaded, now, so it
+      // arrives during this half's products
+      half8 bhn[2], bln[2];
+      if (kc + KC < kp) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int ar = (wm * 128 + t * 32 + fr) * KB + ch;
-        ah[kk][t] = *reinterpret_cast<const half8 *>(&sa[st][0][ar]);
-        al[kk][t] = *reinterpret_cast<const half8 *>(&sa[st][1][ar]);
+        for (int kk = 0; kk < 2; ++kk) {
+          bhn[kk] = load_b(rbh, kc + KC, 0, kk);
+          bln[kk] = load_b(rbl, kc + KC, 0, kk);
+        }
       }
-    }
-    if (k0 + KB < kp) stage(k0 + KB, st ^ 1);
-    // this step's T fragments are in registers.  T is Toeplitz and KB is
-    // the tile width, so the next step's tile 1 is this step's tile 0
-    // (T[n + 32][k + 32] = T[n][k]); only its tile 0 is loaded, now, so it
-    // arrives during this step's products
-    half8 bhn[2], bln[2];
-    if (k0 + KB < kp) {
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk) {
-        bhn[kk] = load_b(rbh, k0 + KB, 0, kk);
-        bln[kk] = load_b(rbl, k0 + KB, 0, kk);
-      }
-    }
+        // one 16-wide slice's fragments, then its products (registers)
+        const int ch = ((2 * (2 * h + kk) + fk) ^ sw) * 8;   // halves
+        half8 ah[4], al[4];
 #pragma unroll
-    for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[kk][a], bh[kk][b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[kk][a], bl[kk][b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[kk][a], bh[kk][b], acc[a][b], 0, 0, 0);
+        for (int t = 0; t < 4; ++t) {
+          const int ar = (wm * 128 + t * 32 + fr) * KB + ch;
+          ah[t] = *reinterpret_cast<const half8 *>(&sa[st][0][ar]);
+          al[t] = *reinterpret_cast<const half8 *>(&sa[st][1][ar]);
         }
+        // the next step's copies go out after this step's last fragment
+        // reads (the compiler makes LDS reads that follow a direct copy wait
+        // for it, whatever the buffer)
+        if (h == 1 && kk == 1 && k0 + KB < kp) stage(k0 + KB, st ^ 1);
 #pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
-      bh[kk][1] = bh[kk][0];
-      bl[kk][1] = bl[kk][0];
-      bh[kk][0] = bhn[kk];
-      bl[kk][0] = bln[kk];
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh[kk][b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl[kk][b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh[kk][b], acc[a][b], 0, 0, 0);
+          }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        bh[kk][1] = bh[kk][0];
+        bl[kk][1] = bl[kk][0];
+        bh[kk][0] = bhn[kk];
+        bl[kk][0] = bln[kk];
+      }
     }
   }
   // D layout: lane holds column n = lane % 32, rows 8(i/4) + 4(lane/32) + i%4
