@@ -82,6 +82,8 @@ struct cbct_ctx_s {
   unsigned short *d_tlo = nullptr; // fp16 lo parts, the same layout
   int toe_ld = 0;                // length of one copy (halves)
   float toe_inv_scale = 1.0f;    // inverse of the power-of-two scale applied to them
+  float *d_fft = nullptr;        // FFT engine: twiddles W_N^m (N complex), then C[k] / N (N floats)
+  int fft_log2n = 0;             // log2 N of those tables
   // cosine weights cache
   float *d_cosw = nullptr;
   cbct_geom cosw_geom{};

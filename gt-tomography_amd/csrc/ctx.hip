@@ -249,7 +249,8 @@ int cbct_ctx_destroy(cbct_ctx ctx) {
   for (auto &pair : ctx->ev)
     for (auto &e : pair)
       if (e) (void)hipEventDestroy(e);
-  void *ptrs[] = {ctx->d_views_d, ctx->d_views_f, ctx->d_angles, ctx->d_h, ctx->d_thi, ctx->d_tlo, ctx->d_cosw};
+  void *ptrs[] = {ctx->d_views_d, ctx->d_views_f, ctx->d_angles, ctx->d_h,
+                  ctx->d_thi,     ctx->d_tlo,     ctx->d_fft,    ctx->d_cosw};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);

@@ -11,6 +11,7 @@
 // That is a dense (rows x Nu) * (Nu x Nu) product -- GEMM-shaped work, so it
 // runs as a tiled contraction (FP32 VALU engine, or FP16 hi/lo split MFMA
 // engine with FP32 accumulation).
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -536,6 +537,343 @@ int run_toeplitz_mfma(cbct_ctx ctx, int nu, int nv, size_t nrows, const float *i
   return CBCT_OK;
 }
 
+// ------------------------------------------------------------- FFT engine
+// The convolution evaluated as the reference does it, by FFT, two rows at a
+// time.  With c[t] = h[|t|] for |t| < nu (circular, zero elsewhere) and its
+// real, even spectrum C over N = 2^L >= 2nu - 1 points, the circular
+// convolution of a row zero padded to N is the Toeplitz product on the first
+// nu outputs.  C is real, so rows a and b go through one complex transform as
+// z = a / s_a + i b / s_b (s: a power of two near the row's L2 norm, so that
+// neither row's rounding swamps the other's), and, since
+// IFFT(Z C) = conj(FFT(conj(Z) C / N)), both transforms are forward ones:
+// v = FFT(conj(FFT(z)) C / N), y_a = s_a Re v, y_b = -s_b Im v.
+//
+// Each transform is a Stockham FFT: radix-8 passes, then one radix-2 or -4
+// pass; pass (R, NS) takes butterfly j's inputs X[j + r N/R], multiplies them
+// by W_N^(r (j mod NS) N / (NS R)), and writes its R outputs to
+// Y[(j / NS) NS R + j mod NS + s NS].  N/8 threads per row pair, 8 points per
+// thread; the passes exchange points through LDS (two buffers, one float2 of
+// padding after every 8).  A thread's 8 points sit at j0 + m N/8 both as the
+// first pass's inputs and as the last pass's outputs, so the loads, the
+// spectral product between the transforms and the stores stay in registers
+// and are coalesced.  One workgroup holds max(1, 2048 / N) pairs.
+struct cf {
+  float x, y;
+};
+__device__ __forceinline__ cf operator+(cf a, cf b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cf operator-(cf a, cf b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cf cmul(cf a, cf b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cf mul_mi(cf a) { return {a.y, -a.x}; }   // a * (-i)
+
+__device__ __forceinline__ void dft2(cf &a0, cf &a1) {
+  const cf t = a0 - a1;
+  a0 = a0 + a1;
+  a1 = t;
+}
+// natural order in and out: y_s = sum_r a_r (-i)^(r s)
+__device__ __forceinline__ void dft4(cf &a0, cf &a1, cf &a2, cf &a3) {
+  const cf t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = mul_mi(a1 - a3);
+  a0 = t0 + t2;
+  a1 = t1 + t3;
+  a2 = t0 - t2;
+  a3 = t1 - t3;
+}
+// y_s = sum_r a_r W8^(r s), W8 = exp(-i pi / 4): even and odd halves by dft4
+__device__ __forceinline__ void dft8(cf (&a)[8]) {
+  cf e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6], o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4(e0, e1, e2, e3);
+  dft4(o0, o1, o2, o3);
+  const float r = 0.70710678118654752f;
+  o1 = {r * (o1.x + o1.y), r * (o1.y - o1.x)};    // * W8   = (1 - i) / sqrt 2
+  o2 = mul_mi(o2);                                 // * W8^2 = -i
+  o3 = {r * (o3.y - o3.x), -r * (o3.x + o3.y)};   // * W8^3 = (-1 - i) / sqrt 2
+  a[0] = e0 + o0;
+  a[1] = e1 + o1;
+  a[2] = e2 + o2;
+  a[3] = e3 + o3;
+  a[4] = e0 - o0;
+  a[5] = e1 - o1;
+  a[6] = e2 - o2;
+  a[7] = e3 - o3;
+}
+
+__device__ __forceinline__ int fft_pad(int i) { return i + (i >> 3); }
+
+// LDS barrier: waits for this wave's LDS accesses only, so global loads
+// issued earlier (the next pair's rows) stay in flight across it
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <int N, int NS>
+struct FftRadix {
+  static constexpr int R = N / NS >= 8 ? 8 : N / NS;
+};
+
+// twiddles one transform uses, per thread: (8/R)(R-1) for each pass with NS > 1
+template <int N, int NS>
+struct FftTw {
+  static constexpr int R = FftRadix<N, NS>::R;
+  static constexpr int here = NS > 1 ? (8 / R) * (R - 1) : 0;
+  static constexpr int value = here + (NS * R < N ? FftTw<N, NS * R>::value : 0);
+};
+template <int N>
+struct FftTw<N, N> {
+  static constexpr int here = 0, value = 0;
+};
+
+// the thread's twiddles from the table, in the order the passes use them:
+// pass (R, NS), butterfly j = j0 + q N/8, input r: W_N^(r (j mod NS) N/(NS R))
+template <int N, int NS, int OFF, int NT>
+__device__ __forceinline__ void fft_tw_load(cf (&w)[NT], int j0, const cf *__restrict__ tw) {
+  constexpr int R = FftRadix<N, NS>::R;
+  if constexpr (NS > 1) {
+#pragma unroll
+    for (int q = 0; q < 8 / R; ++q) {
+      const int k = (j0 + q * (N / 8)) & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[OFF + q * (R - 1) + r - 1] = tw[k * r * (N / (NS * R))];
+    }
+  }
+  if constexpr (NS * R < N) fft_tw_load<N, NS * R, OFF + FftTw<N, NS>::here, NT>(w, j0, tw);
+}
+
+// pass (R, NS) on the thread's butterflies j = j0 + q N/8, q < 8/R, whose
+// inputs are a[q R + r]; the outputs replace them as a[q R + s]
+template <int N, int NS, int OFF, int NT>
+__device__ __forceinline__ void fft_pass(cf (&a)[8], const cf (&w)[NT]) {
+  constexpr int R = FftRadix<N, NS>::R;
+#pragma unroll
+  for (int q = 0; q < 8 / R; ++q) {
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) a[q * R + r] = cmul(a[q * R + r], w[OFF + q * (R - 1) + r - 1]);
+    }
+    if constexpr (R == 8) dft8(a);
+    else if constexpr (R == 4) dft4(a[q * 4], a[q * 4 + 1], a[q * 4 + 2], a[q * 4 + 3]);
+    else dft2(a[q * 2], a[q * 2 + 1]);
+  }
+}
+
+// passes from sub-transform size NS on; a[] holds that pass's inputs, and on
+// return the last pass's outputs.  Writes alternate between X and Y.  LDS
+// positions are padded as p + p / 8; every offset added to a padded base
+// below is a multiple of 8 (or the base is 8 j0 with offsets < 8), so it
+// pads to a constant and the accesses are one base register plus immediates.
+template <int N, int NS, int OFF, int NT>
+__device__ __forceinline__ void fft_from(cf (&a)[8], int j0, const cf (&w)[NT], cf *X, cf *Y) {
+  constexpr int R = FftRadix<N, NS>::R;
+  fft_pass<N, NS, OFF, NT>(a, w);
+  if constexpr (NS * R < N) {
+    constexpr int R2 = FftRadix<N, NS * R>::R;
+    if constexpr (NS == 1) {   // R = 8, one butterfly j0: positions 8 j0 + s
+      cf *x = X + 9 * j0;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) x[s] = a[s];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8 / R; ++q) {
+        const int j = j0 + q * (N / 8);
+        cf *x = X + fft_pad((j & ~(NS - 1)) * R + (j & (NS - 1)));
+#pragma unroll
+        for (int s = 0; s < R; ++s) x[s * NS + s * NS / 8] = a[q * R + s];
+      }
+    }
+    lds_barrier();
+    const cf *y = X + fft_pad(j0);
+#pragma unroll
+    for (int q = 0; q < 8 / R2; ++q)
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const int c = q * (N / 8) + r * (N / R2);   // a multiple of 8
+        a[q * R2 + r] = y[c + c / 8];
+      }
+    fft_from<N, NS * R, OFF + FftTw<N, NS>::here, NT>(a, j0, w, Y, X);
+  }
+}
+
+template <int N, int NS>
+struct FftWrites {   // LDS writes (passes - 1) of one transform
+  static constexpr int R = FftRadix<N, NS>::R;
+  static constexpr int value = NS * R < N ? 1 + FftWrites<N, NS * R>::value : 0;
+};
+template <int N>
+struct FftWrites<N, N> {
+  static constexpr int value = 0;
+};
+
+template <int LOG2N>
+struct FftShape {
+  static constexpr int N = 1 << LOG2N;
+  static constexpr int G = N >= 2048 ? 1 : 2048 / N;   // row pairs per workgroup
+  static constexpr int T = G * N / 8;                  // threads
+  static constexpr int RL = N >> (3 * ((LOG2N - 1) / 3));   // radix of the last pass
+};
+
+// Persistent: workgroup b takes pair groups b, b + gridDim.x, ...; the next
+// group's rows (and cosine weights) are loaded while this one transforms.
+template <int LOG2N, bool PARKER>
+__global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
+    const float *in, float *out, size_t nrows, int nu, const cf *__restrict__ tw,
+    const float *__restrict__ cs, const float *__restrict__ cosw, int nv,
+    const float *__restrict__ angles, float delta, float tan_delta) {
+  using S = FftShape<LOG2N>;
+  constexpr int N = S::N, N8 = N / 8, G = S::G, T = S::T, RL = S::RL;
+  constexpr int PN = N + N / 8;   // padded length of one pair's buffer
+  constexpr int NW = T / 64 > 0 ? T / 64 : 1;
+  constexpr int NT = FftTw<N, 1>::value;
+  __shared__ cf buf[2][G * PN];
+  __shared__ double part[2][NW];
+  const int t = threadIdx.x, g = t / N8, j0 = t & (N8 - 1);
+  const size_t ngroups = (nrows + 2 * G - 1) / (2 * G);
+
+  cf w[NT];
+  fft_tw_load<N, 1, 0, NT>(w, j0, tw);
+  float csr[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) csr[m] = cs[j0 + m * N8];
+
+  // Rows go through buffer resources over one group's 2G rows (base and
+  // size uniform, 32-bit offsets): reads past the group's last row or past
+  // nu return 0 and such writes are dropped, so no load or store sits under
+  // a branch and the memory counters stay exact across iterations (loads
+  // and stores share one counter, so an uncertain count would make every
+  // iteration wait for its stores).
+  auto rsrc = [](const float *p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+  };
+  auto group_rows = [&](size_t gi) -> size_t {
+    const size_t r0 = gi * 2 * G;
+    return r0 >= nrows ? 0 : (nrows - r0 < 2 * (size_t)G ? nrows - r0 : 2 * (size_t)G);
+  };
+  constexpr unsigned OOB = 0x80000000u;   // beyond any group's bytes
+  const __amdgpu_buffer_rsrc_t rcos = rsrc(cosw, cosw ? (size_t)nv * nu * sizeof(float) : 0);
+  // raw rows and cosine weights of group gi; positions j0 + m N/8 for m >= 4
+  // are >= N/2 >= nu
+  float x[2][4], cwv[2][4];
+  auto fetch = [&](size_t gi) {
+    const __amdgpu_buffer_rsrc_t rin =
+        rsrc(in + gi * 2 * G * (size_t)nu, group_rows(gi) * nu * sizeof(float));
+    const unsigned v0 = (unsigned)((gi * 2 * G) % (size_t)nv);   // uniform
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int rl = 2 * g + h;   // row within the group
+      const unsigned v = (v0 + (unsigned)rl) % (unsigned)nv;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int e = j0 + m * N8;
+        const unsigned off = e < nu ? (unsigned)(rl * nu + e) * 4u : OOB;
+        const unsigned coff = e < nu ? (v * (unsigned)nu + (unsigned)e) * 4u : OOB;
+        x[h][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, off, 0, 0));
+        cwv[h][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rcos, coff, 0, 0));
+      }
+    }
+  };
+  // one group: consume its prefetched rows, fetch the next group's, transform,
+  // store.  The first group is peeled off the loop, so that every entry to
+  // the loop follows a group's stores, as the back edge does: the memory
+  // counters then agree at the loop head and waiting for the prefetched
+  // rows never waits for the previous group's stores (loads and stores
+  // share one in-order counter).
+  auto process = [&](size_t gi) __attribute__((always_inline)) {
+    const size_t ra = (gi * G + g) * 2;
+    const __amdgpu_buffer_rsrc_t rout =
+        rsrc(out + gi * 2 * G * (size_t)nu, group_rows(gi) * nu * sizeof(float));
+    // the pair's weighted rows and their sums of squares
+    float xv[2][4];
+    double qs[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const size_t r = ra + h;
+      float beta = 0.0f;
+      if (PARKER && r < nrows) beta = angles[r / (size_t)nv] * (3.14159265358979f / 180.0f);
+      double q = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        float v = x[h][m];
+        if (cosw) v *= cwv[h][m];
+        if (PARKER) v *= parker_weight(j0 + m * N8, nu, beta, delta, tan_delta);
+        xv[h][m] = v;
+        q += (double)v * (double)v;
+      }
+      qs[h] = q;
+    }
+    fetch(gi + gridDim.x);
+    // sums of squares over the pair's N/8 threads (within one wave up to 64)
+#pragma unroll
+    for (int o = (N8 < 64 ? N8 : 64) / 2; o > 0; o >>= 1) {
+      qs[0] += __shfl_xor(qs[0], o);
+      qs[1] += __shfl_xor(qs[1], o);
+    }
+    if constexpr (N8 > 64) {
+      if ((t & 63) == 0) {
+        part[0][t >> 6] = qs[0];
+        part[1][t >> 6] = qs[1];
+      }
+      lds_barrier();
+      qs[0] = qs[1] = 0.0;
+#pragma unroll
+      for (int wv = 0; wv < N8 / 64; ++wv) {
+        qs[0] += part[0][g * (N8 / 64) + wv];
+        qs[1] += part[1][g * (N8 / 64) + wv];
+      }
+    }
+    // power-of-two scales: |row / s| in [1, 2); zero rows give zero
+    float sc[2], inv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (qs[h] > 0.0) {
+        int e = 0;
+        (void)frexp(sqrt(qs[h]), &e);
+        e = e - 1 < -120 ? -120 : (e - 1 > 120 ? 120 : e - 1);
+        sc[h] = ldexpf(1.0f, e);
+        inv[h] = ldexpf(1.0f, -e);
+      } else {
+        sc[h] = inv[h] = 0.0f;
+      }
+    }
+    cf a[8];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) a[m] = {xv[0][m] * inv[0], xv[1][m] * inv[1]};
+#pragma unroll
+    for (int m = 4; m < 8; ++m) a[m] = {0.0f, 0.0f};
+    cf *X = &buf[0][g * PN], *Y = &buf[1][g * PN];
+    fft_from<N, 1, 0, NT>(a, j0, w, X, Y);
+    // a[q RL + s] = Z[j0 + (q + s 8/RL) N/8]: the spectral product, conjugated,
+    // into the first pass's order b[m] = V[j0 + m N/8]
+    cf b[8];
+#pragma unroll
+    for (int q = 0; q < 8 / RL; ++q)
+#pragma unroll
+      for (int s = 0; s < RL; ++s) {
+        const int m = q + s * (8 / RL);
+        const cf z = a[q * RL + s];
+        b[m] = {z.x * csr[m], -z.y * csr[m]};
+      }
+    // the next write goes to the buffer written last but one (a slower wave
+    // may still read the last one); each transform writes FftWrites times,
+    // so every group starts on X again
+    constexpr bool odd = FftWrites<N, 1>::value & 1;
+    fft_from<N, 1, 0, NT>(b, j0, w, odd ? Y : X, odd ? X : Y);
+#pragma unroll
+    for (int q = 0; q < 8 / RL; ++q)
+#pragma unroll
+      for (int s = 0; s < RL; ++s) {
+        const int e = j0 + (q + s * (8 / RL)) * N8;
+        const cf v = b[q * RL + s];
+        const unsigned oa = e < nu ? (unsigned)(2 * g * nu + e) * 4u : OOB;
+        const unsigned ob = e < nu ? (unsigned)((2 * g + 1) * nu + e) * 4u : OOB;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v.x * sc[0]), rout, oa, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, -v.y * sc[1]), rout, ob, 0, 0);
+      }
+  };
+  fetch(blockIdx.x);
+  if ((size_t)blockIdx.x < ngroups) {
+    process(blockIdx.x);
+    for (size_t gi = (size_t)blockIdx.x + gridDim.x; gi < ngroups; gi += gridDim.x) process(gi);
+  }
+}
+
 // Host: one-sided response F[0..nu] -> spatial kernel h[0..nu-1] (FP64).
 std::vector<double> spatial_kernel(int nu, const float *F) {
   std::vector<double> h(nu);
@@ -569,6 +907,119 @@ int ensure_cos_weights(cbct_ctx ctx, const cbct_geom *g) {
   if (rc) return rc;
   ctx->cosw_geom = *g;
   ctx->cosw_valid = true;
+  return CBCT_OK;
+}
+
+constexpr int FFT_MIN_LOG2N = 6, FFT_MAX_LOG2N = 13;   // N = 64 .. 8192, nu <= 4096
+
+// log2 of the FFT engine's transform length for rows of nu: N >= 2nu - 1
+int fft_log2n(int nu) {
+  int l = FFT_MIN_LOG2N;
+  while ((1 << l) < 2 * nu - 1) ++l;
+  return l;
+}
+
+// Host: the FFT engine's tables for the current filter, from its FP64
+// kernel: twiddles W_N^m = exp(-2 pi i m / N), m < N, and the spectrum of
+// the circular kernel c[t] = h[|t|] (|t| < nu), C[k] = h[0] + 2 sum_t h[t]
+// cos(2 pi k t / N), divided by N.  Built once per filter on first use.
+int ensure_fft_tables(cbct_ctx ctx, int log2n) {
+  if (ctx->d_fft && ctx->fft_log2n == log2n) return CBCT_OK;
+  const int nu = ctx->filter_nu;
+  const size_t N = size_t(1) << log2n;
+  const double pi = 3.14159265358979323846;
+  std::vector<double> cs(N);
+  std::vector<float> tab(3 * N);
+  for (size_t m = 0; m < N; ++m) {
+    cs[m] = std::cos(2.0 * pi * (double)m / (double)N);
+    tab[2 * m] = (float)cs[m];
+    tab[2 * m + 1] = (float)-std::sin(2.0 * pi * (double)m / (double)N);
+  }
+  const std::vector<double> &h = ctx->h64;
+  for (size_t k = 0; k <= N / 2; ++k) {
+    double acc = h[0];
+    for (int t = 1; t < nu; ++t) acc += 2.0 * h[t] * cs[(k * (size_t)t) & (N - 1)];
+    tab[2 * N + k] = (float)(acc / (double)N);
+    tab[2 * N + ((N - k) & (N - 1))] = tab[2 * N + k];
+  }
+  if (ctx->d_fft) {
+    int rc = cbct_wait_tables(ctx);
+    if (rc) return rc;
+    (void)hipFree(ctx->d_fft);
+    ctx->d_fft = nullptr;
+  }
+  CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_fft, sizeof(float) * tab.size()));
+  int rc = cbct_upload(ctx, ctx->d_fft, tab.data(), sizeof(float) * tab.size());
+  if (rc) return rc;
+  ctx->fft_log2n = log2n;
+  return CBCT_OK;
+}
+
+// workgroups of k_ramp_fft<L> that fit on the device at once (its grid)
+std::atomic<int> g_fft_resident[2][FFT_MAX_LOG2N + 1];   // [parker][log2 N]
+
+template <int L>
+int launch_ramp_fft(cbct_ctx ctx, size_t ngroups, hipStream_t s, const float *in, float *out,
+                    size_t nrows, int nu, const float *tab, const float *cosw, int nv,
+                    const float *angles, int parker, float delta, float tan_delta) {
+  const size_t N = size_t(1) << L;
+  const int pk = parker ? 1 : 0;
+  int resident = g_fft_resident[pk][L].load();
+  if (resident == 0) {
+    int per_cu = 0, cus = 0;
+    if (parker)
+      CBCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ramp_fft<L, true>,
+                                                                        FftShape<L>::T, 0));
+    else
+      CBCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ramp_fft<L, false>,
+                                                                        FftShape<L>::T, 0));
+    CBCT_HIP_CHECK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    resident = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    g_fft_resident[pk][L].store(resident);
+  }
+  const unsigned nb = (unsigned)(ngroups < (size_t)resident ? ngroups : (size_t)resident);
+  // the short-scan weights (Parker) only in their own instantiation: their
+  // range reductions would otherwise set the common kernel's register count
+  if (parker)
+    hipLaunchKernelGGL((k_ramp_fft<L, true>), dim3(nb), dim3(FftShape<L>::T), 0, s, in, out, nrows,
+                       nu, reinterpret_cast<const cf *>(tab), tab + 2 * N, cosw, nv, angles, delta,
+                       tan_delta);
+  else
+    hipLaunchKernelGGL((k_ramp_fft<L, false>), dim3(nb), dim3(FftShape<L>::T), 0, s, in, out, nrows,
+                       nu, reinterpret_cast<const cf *>(tab), tab + 2 * N, cosw, nv, angles, delta,
+                       tan_delta);
+  return CBCT_OK;
+}
+
+// Weighted (cosine plane, Parker; none when cosw is null) rows -> FFT-engine
+// filter into out.  Each workgroup reads its rows whole before it writes
+// them, so in may alias out.  Rows longer than 4096 go to the MFMA engine.
+int run_ramp_fft(cbct_ctx ctx, int nu, int nv, size_t nrows, const float *in, float *out,
+                 const float *cosw, const float *angles, int parker, float delta, float tan_delta,
+                 hipStream_t s) {
+  if (nrows == 0) return CBCT_OK;
+  const int L = fft_log2n(nu);
+  if (L > FFT_MAX_LOG2N)
+    return run_toeplitz_mfma(ctx, nu, nv, nrows, in, out, cosw, angles, parker, delta, tan_delta, s);
+  int rc = ensure_fft_tables(ctx, L);
+  if (rc) return rc;
+  const size_t rows_per_group = 2 * (size_t)(L >= 11 ? 1 : 2048 >> L);
+  const size_t ng = (nrows + rows_per_group - 1) / rows_per_group;
+  const float *tab = ctx->d_fft;
+  cbct_prof_mark(ctx, PROF_FILTER, 0, s);
+  switch (L) {
+    case 6: rc = launch_ramp_fft<6>(ctx, ng, s, in, out, nrows, nu, tab, cosw, nv, angles, parker, delta, tan_delta); break;
+    case 7: rc = launch_ramp_fft<7>(ctx, ng, s, in, out, nrows, nu, tab, cosw, nv, angles, parker, delta, tan_delta); break;
+    case 8: rc = launch_ramp_fft<8>(ctx, ng, s, in, out, nrows, nu, tab, cosw, nv, angles, parker, delta, tan_delta); break;
+    case 9: rc = launch_ramp_fft<9>(ctx, ng, s, in, out, nrows, nu, tab, cosw, nv, angles, parker, delta, tan_delta); break;
+    case 10: rc = launch_ramp_fft<10>(ctx, ng, s, in, out, nrows, nu, tab, cosw, nv, angles, parker, delta, tan_delta); break;
+    case 11: rc = launch_ramp_fft<11>(ctx, ng, s, in, out, nrows, nu, tab, cosw, nv, angles, parker, delta, tan_delta); break;
+    case 12: rc = launch_ramp_fft<12>(ctx, ng, s, in, out, nrows, nu, tab, cosw, nv, angles, parker, delta, tan_delta); break;
+    default: rc = launch_ramp_fft<13>(ctx, ng, s, in, out, nrows, nu, tab, cosw, nv, angles, parker, delta, tan_delta); break;
+  }
+  if (rc) return rc;
+  cbct_prof_mark(ctx, PROF_FILTER, 1, s);
+  CBCT_LAUNCH_CHECK(ctx, "k_ramp_fft");
   return CBCT_OK;
 }
 
@@ -642,8 +1093,10 @@ int cbct_filter_set(cbct_ctx ctx, int nu, const float *F) {
   if (ctx->d_h) (void)hipFree(ctx->d_h);
   if (ctx->d_thi) (void)hipFree(ctx->d_thi);
   if (ctx->d_tlo) (void)hipFree(ctx->d_tlo);
+  if (ctx->d_fft) (void)hipFree(ctx->d_fft);
   ctx->d_h = nullptr;
   ctx->d_thi = ctx->d_tlo = nullptr;
+  ctx->d_fft = nullptr;
   ctx->filter_nu = 0;
   ctx->h64 = h;
   CBCT_HIP_CHECK(ctx, hipMalloc(&ctx->d_h, sizeof(float) * hm.size()));
@@ -655,7 +1108,7 @@ int cbct_filter_set(cbct_ctx ctx, int nu, const float *F) {
 
 int cbct_filter_engine(cbct_ctx ctx, int engine) {
   if (!ctx) return CBCT_ERR_ARG;
-  if (engine != 0 && engine != 1)
+  if (engine < 0 || engine > 2)
     return cbct_fail(ctx, CBCT_ERR_UNSUPPORTED, "cbct_filter_engine: engine %d not available", engine);
   ctx->filter_engine = engine;
   return CBCT_OK;
@@ -671,6 +1124,9 @@ int cbct_ramp_filter(cbct_ctx ctx, int nu, size_t nrows, const float *in, float 
   int rc = cbct_set_device(ctx);
   if (rc) return rc;
   CallScope scope(ctx, as_stream(stream));
+  if (ctx->filter_engine == 2)
+    return run_ramp_fft(ctx, nu, 1, nrows, in, out, nullptr, nullptr, 0, 0.0f, 0.0f,
+                        as_stream(stream));
   if (ctx->filter_engine == 1)
     return run_toeplitz_mfma(ctx, nu, 1, nrows, in, out, nullptr, nullptr, 0, 0.0f, 0.0f,
                              as_stream(stream));
@@ -702,7 +1158,12 @@ int cbct_fdk_prefilter(cbct_ctx ctx, const cbct_geom *g, int view0, int nviews, 
   const float delta = std::atan(g->fov_u / (2.0f * g->sdd));
   const float tan_delta = std::tan(delta);
   const size_t nrows = (size_t)g->nv * nviews;
-  if (ctx->filter_engine == 1) {
+  if (ctx->filter_engine == 2) {
+    // weights fused into the transform's loads
+    rc = run_ramp_fft(ctx, g->nu, g->nv, nrows, in, out, ctx->d_cosw, d_angles,
+                      short_scan ? 1 : 0, delta, tan_delta, s);
+    if (rc) return rc;
+  } else if (ctx->filter_engine == 1) {
     // weights fused into the fp16 split, then the MFMA product into out
     rc = run_toeplitz_mfma(ctx, g->nu, g->nv, nrows, in, out, ctx->d_cosw, d_angles,
                            short_scan ? 1 : 0, delta, tan_delta, s);

@@ -131,7 +131,7 @@ def test_fdk128_bp_full(ctx, fdk128, fbp):
         assert torch.equal(outs[3], outs[3 + i]), 3 + i
 
 
-@pytest.mark.parametrize("engine", [0, 1])
+@pytest.mark.parametrize("engine", [0, 1, 2])
 def test_fdk128_prefilter_full(ctx, fdk128, engine):
     cfg, g, angles, vol, volh, p_ref = fdk128
     print_case("fdk128 prefilter engine", engine)
@@ -375,11 +375,11 @@ def test_prec512_fbp_subsample(ctx, prec512):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("engine", [1, 0])
+@pytest.mark.parametrize("engine", [1, 2, 0])
 def test_prec512_prefilter_real_rows(ctx, prec512, engine):
     """The FDK pre-filter of the whole 720-view sinogram (552,960 rows of
     1024); every 90th view checked against the oracle's exact Toeplitz
-    filter.  Engine 1 is the FP16 hi/lo split MFMA product."""
+    filter.  Engine 1 is the FP16 hi/lo split MFMA product, engine 2 the FFT."""
     d = prec512
     g, angles, sel = d["g"], d["angles"], d["sel_views"]
     print_case("prec512 prefilter engine", engine)

@@ -364,7 +364,7 @@ def test_fp_transpose_parity_and_adjoint(ctx):
     assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
 
 
-FILTER_ENGINES = [0, 1]   # 0: FP32 VALU Toeplitz, 1: FP16 hi/lo split MFMA
+FILTER_ENGINES = [0, 1, 2]   # 0: FP32 VALU Toeplitz, 1: FP16 hi/lo split MFMA, 2: FFT
 
 
 @pytest.mark.parametrize("engine", FILTER_ENGINES)
