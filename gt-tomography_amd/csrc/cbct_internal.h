@@ -62,7 +62,7 @@ struct cbct_ctx_s {
   bool done_recorded = false;
   // filter state
   int filter_nu = 0;
-  int filter_engine = 1;
+  int filter_engine = 2;
   int fp_engine = 1;             // 1: detector-column sharing, 0: one thread per ray
   int bp_engine = 2;             // 2: LDS-staged, 3: LDS-staged forced (BF16), 1: quad, 0: plain
   // backprojector: whether the LDS tile of the BF16 pair kernel holds the

@@ -8,9 +8,10 @@
 // convolution never wraps, and the whole FFT chain is exactly the Toeplitz
 // product  y[i] = sum_j h[|i-j|] x[j]  with
 //   h[m] = (F0 + 2 sum_{k=1}^{Nu-1} Fk cos(pi k m / Nu) + F_Nu (-1)^m) / (2Nu).
-// That is a dense (rows x Nu) * (Nu x Nu) product -- GEMM-shaped work, so it
-// runs as a tiled contraction (FP32 VALU engine, or FP16 hi/lo split MFMA
-// engine with FP32 accumulation).
+// That is a dense (rows x Nu) * (Nu x Nu) product, run as a tiled
+// contraction (FP32 VALU engine, or FP16 hi/lo split MFMA engine with FP32
+// accumulation), or, like the reference, by FFT (the default engine: two rows
+// per complex transform, N >= 2Nu - 1, so the convolution never wraps).
 #include <atomic>
 #include <cmath>
 #include <vector>
@@ -610,11 +611,18 @@ struct FftRadix {
   static constexpr int R = N / NS >= 8 ? 8 : N / NS;
 };
 
-// twiddles one transform uses, per thread: (8/R)(R-1) for each pass with NS > 1
+// twiddles a butterfly keeps: W^1, W^2, W^4 for radix 8 (W^3, W^5, W^6, W^7
+// are products of those, at most two deep), W^1, W^2 for radix 4, W^1 for 2
+template <int R>
+struct FftKeep {
+  static constexpr int value = R == 8 ? 3 : (R == 4 ? 2 : 1);
+};
+
+// twiddles one transform keeps per thread: (8/R) FftKeep<R> per pass with NS > 1
 template <int N, int NS>
 struct FftTw {
   static constexpr int R = FftRadix<N, NS>::R;
-  static constexpr int here = NS > 1 ? (8 / R) * (R - 1) : 0;
+  static constexpr int here = NS > 1 ? (8 / R) * FftKeep<R>::value : 0;
   static constexpr int value = here + (NS * R < N ? FftTw<N, NS * R>::value : 0);
 };
 template <int N>
@@ -622,32 +630,55 @@ struct FftTw<N, N> {
   static constexpr int here = 0, value = 0;
 };
 
-// the thread's twiddles from the table, in the order the passes use them:
-// pass (R, NS), butterfly j = j0 + q N/8, input r: W_N^(r (j mod NS) N/(NS R))
+// the thread's kept twiddles from the table, in the order the passes use
+// them: pass (R, NS), butterfly j = j0 + q N/8: W_N^(p (j mod NS) N/(NS R))
+// for p = 1, 2, 4 (those below R)
 template <int N, int NS, int OFF, int NT>
 __device__ __forceinline__ void fft_tw_load(cf (&w)[NT], int j0, const cf *__restrict__ tw) {
-  constexpr int R = FftRadix<N, NS>::R;
+  constexpr int R = FftRadix<N, NS>::R, K = FftKeep<R>::value;
   if constexpr (NS > 1) {
 #pragma unroll
     for (int q = 0; q < 8 / R; ++q) {
       const int k = (j0 + q * (N / 8)) & (NS - 1);
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[OFF + q * (R - 1) + r - 1] = tw[k * r * (N / (NS * R))];
+      for (int i = 0; i < K; ++i) w[OFF + q * K + i] = tw[k * (1 << i) * (N / (NS * R))];
     }
   }
   if constexpr (NS * R < N) fft_tw_load<N, NS * R, OFF + FftTw<N, NS>::here, NT>(w, j0, tw);
 }
 
+// keeps the derived twiddles' products inside the loop that uses them (the
+// compiler would otherwise hoist them and hold all seven per butterfly)
+__device__ __forceinline__ void fft_pin(cf &w) { asm volatile("" : "+v"(w.x), "+v"(w.y)); }
+
 // pass (R, NS) on the thread's butterflies j = j0 + q N/8, q < 8/R, whose
 // inputs are a[q R + r]; the outputs replace them as a[q R + s]
 template <int N, int NS, int OFF, int NT>
-__device__ __forceinline__ void fft_pass(cf (&a)[8], const cf (&w)[NT]) {
-  constexpr int R = FftRadix<N, NS>::R;
+__device__ __forceinline__ void fft_pass(cf (&a)[8], cf (&w)[NT]) {
+  constexpr int R = FftRadix<N, NS>::R, K = FftKeep<R>::value;
 #pragma unroll
   for (int q = 0; q < 8 / R; ++q) {
     if constexpr (NS > 1) {
+      cf *wq = &w[OFF + q * K];
 #pragma unroll
-      for (int r = 1; r < R; ++r) a[q * R + r] = cmul(a[q * R + r], w[OFF + q * (R - 1) + r - 1]);
+      for (int i = 0; i < K; ++i) fft_pin(wq[i]);
+      if constexpr (R == 8) {
+        const cf w1 = wq[0], w2 = wq[1], w4 = wq[2], w3 = cmul(w1, w2);
+        a[1] = cmul(a[1], w1);
+        a[2] = cmul(a[2], w2);
+        a[3] = cmul(a[3], w3);
+        a[4] = cmul(a[4], w4);
+        a[5] = cmul(a[5], cmul(w1, w4));
+        a[6] = cmul(a[6], cmul(w2, w4));
+        a[7] = cmul(a[7], cmul(w3, w4));
+      } else if constexpr (R == 4) {
+        const cf w1 = wq[0], w2 = wq[1];
+        a[q * 4 + 1] = cmul(a[q * 4 + 1], w1);
+        a[q * 4 + 2] = cmul(a[q * 4 + 2], w2);
+        a[q * 4 + 3] = cmul(a[q * 4 + 3], cmul(w1, w2));
+      } else {
+        a[q * 2 + 1] = cmul(a[q * 2 + 1], wq[0]);
+      }
     }
     if constexpr (R == 8) dft8(a);
     else if constexpr (R == 4) dft4(a[q * 4], a[q * 4 + 1], a[q * 4 + 2], a[q * 4 + 3]);
@@ -661,7 +692,7 @@ __device__ __forceinline__ void fft_pass(cf (&a)[8], const cf (&w)[NT]) {
 // below is a multiple of 8 (or the base is 8 j0 with offsets < 8), so it
 // pads to a constant and the accesses are one base register plus immediates.
 template <int N, int NS, int OFF, int NT>
-__device__ __forceinline__ void fft_from(cf (&a)[8], int j0, const cf (&w)[NT], cf *X, cf *Y) {
+__device__ __forceinline__ void fft_from(cf (&a)[8], int j0, cf (&w)[NT], cf *X, cf *Y) {
   constexpr int R = FftRadix<N, NS>::R;
   fft_pass<N, NS, OFF, NT>(a, w);
   if constexpr (NS * R < N) {

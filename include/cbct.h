@@ -250,7 +250,7 @@ int cbct_default_filter(int nu, float *filter_host);
 int cbct_filter_set(cbct_ctx ctx, int nu, const float *filter_host);
 
 /* Ramp-filter engine selection: 0 = fp32 VALU Toeplitz, 1 = fp16 hi/lo split
- * MFMA Toeplitz with fp32 accumulation (default), 2 = FFT (the reference's
+ * MFMA Toeplitz with fp32 accumulation, 2 = FFT (default; the reference's
  * method: two rows per complex transform of length 2^k >= 2nu - 1, fp32 with
  * fp64-derived twiddles and spectrum; rows longer than 4096 use engine 1).
  * Other values: CBCT_ERR_UNSUPPORTED. */

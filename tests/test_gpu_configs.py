@@ -146,7 +146,7 @@ def test_fdk128_prefilter_full(ctx, fdk128, engine):
         out = torch.empty_like(pd)
         ctx.fdk_prefilter(g, pd, out)
     finally:
-        ctx.filter_engine(1)
+        ctx.filter_engine(2)
     err = rel_l2(host64(out), ref)
     print("fdk128 prefilter rel-L2 vs oracle", engine, err)
     assert err <= TOL
@@ -393,7 +393,7 @@ def test_prec512_prefilter_real_rows(ctx, prec512, engine):
         got = out[torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
         del out
     finally:
-        ctx.filter_engine(1)
+        ctx.filter_engine(2)
     ph = d["p"][torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
     ref = O.fdk_prefilter(to_oracle(g), angles[sel], None, ph, F)
     err = rel_l2(got, ref)
