@@ -433,6 +433,30 @@ def test_ramp_filter_dynamic_range(ctx, engine):
     assert worst <= TOL
 
 
+@pytest.mark.parametrize("nu,nrows", [(1, 3), (2, 1), (3, 5), (33, 65), (1000, 7), (4096, 3),
+                                      (4097, 3)])
+def test_fft_filter_edges(ctx, nu, nrows):
+    """The FFT engine at the edges of its transform lengths: the smallest
+    (N = 64 for nu <= 32, 32 row pairs per workgroup), odd row counts (a
+    pair whose second row is missing), a length that is no power of two,
+    the largest (nu = 4096, N = 8192) and the first row length it hands to
+    the MFMA engine (4097); out of place and in place."""
+    print_case("fft filter edges", nu, nrows)
+    F = O.ramlak_filter(nu)
+    rows = np.random.default_rng(7 * nu + nrows).standard_normal((nrows, nu))
+    ref = O.filter_rows(rows, F.astype(np.float32))
+    ctx.set_filter(F)
+    ctx.filter_engine(2)
+    x = dev(rows)
+    out = torch.empty_like(x)
+    ctx.ramp_filter(nu, x, out)
+    err = rel_l2(host(out), ref)
+    ctx.ramp_filter(nu, x, x)
+    err_in_place = rel_l2(host(x), ref)
+    print("fft filter edges rel-L2", nu, nrows, err, err_in_place)
+    assert err <= TOL and err_in_place <= TOL
+
+
 @pytest.mark.parametrize("engine", FILTER_ENGINES)
 @pytest.mark.parametrize("short_scan,offset", [(False, False), (True, False), (False, True)])
 def test_fdk_prefilter_parity(ctx, short_scan, offset, engine):
