@@ -600,8 +600,6 @@ __device__ __forceinline__ void dft8(cf (&a)[8]) {
   a[7] = e3 - o3;
 }
 
-__device__ __forceinline__ int fft_pad(int i) { return i + (i >> 3); }
-
 // LDS barrier: waits for this wave's LDS accesses only, so global loads
 // issued earlier (the next pair's rows) stay in flight across it
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -686,11 +684,23 @@ __device__ __forceinline__ void fft_pass(cf (&a)[8], cf (&w)[NT]) {
   }
 }
 
+// LDS padding of the exchange after pass NS (positions p < N, padded below
+// N + N/8): the pass's writes and the next pass's reads both fall on
+// distinct banks for every 32 lanes (64 banks of 4 bytes, 8-byte points).
+// After the first pass (writes 8 apart): p + p/32; after the second (groups
+// of 8 consecutive, 64 apart): p + 8 (p / 64); later passes write
+// consecutive positions: none.
+template <int NS>
+__device__ __forceinline__ int fft_padx(int p) {
+  if constexpr (NS == 1) return p + (p >> 5);
+  else if constexpr (NS == 8) return p + ((p >> 6) << 3);
+  else return p;
+}
+
 // passes from sub-transform size NS on; a[] holds that pass's inputs, and on
-// return the last pass's outputs.  Writes alternate between X and Y.  LDS
-// positions are padded as p + p / 8; every offset added to a padded base
-// below is a multiple of 8 (or the base is 8 j0 with offsets < 8), so it
-// pads to a constant and the accesses are one base register plus immediates.
+// return the last pass's outputs.  Writes alternate between X and Y.  The
+// padded positions are written as one base register plus constant offsets
+// wherever the offsets allow it.
 template <int N, int NS, int OFF, int NT>
 __device__ __forceinline__ void fft_from(cf (&a)[8], int j0, cf (&w)[NT], cf *X, cf *Y) {
   constexpr int R = FftRadix<N, NS>::R;
@@ -698,26 +708,31 @@ __device__ __forceinline__ void fft_from(cf (&a)[8], int j0, cf (&w)[NT], cf *X,
   if constexpr (NS * R < N) {
     constexpr int R2 = FftRadix<N, NS * R>::R;
     if constexpr (NS == 1) {   // R = 8, one butterfly j0: positions 8 j0 + s
-      cf *x = X + 9 * j0;
+      cf *x = X + 8 * j0 + (j0 >> 2);
 #pragma unroll
       for (int s = 0; s < 8; ++s) x[s] = a[s];
+    } else if constexpr (NS == 8) {   // R = 8: positions 64 (j0 / 8) + j0 % 8 + 8 s
+      cf *x = X + 72 * (j0 >> 3) + (j0 & 7);
+#pragma unroll
+      for (int s = 0; s < 8; ++s) x[8 * s] = a[s];
     } else {
 #pragma unroll
       for (int q = 0; q < 8 / R; ++q) {
         const int j = j0 + q * (N / 8);
-        cf *x = X + fft_pad((j & ~(NS - 1)) * R + (j & (NS - 1)));
+        cf *x = X + (j & ~(NS - 1)) * R + (j & (NS - 1));
 #pragma unroll
-        for (int s = 0; s < R; ++s) x[s * NS + s * NS / 8] = a[q * R + s];
+        for (int s = 0; s < R; ++s) x[s * NS] = a[q * R + s];
       }
     }
     lds_barrier();
-    const cf *y = X + fft_pad(j0);
+    const cf *y = X + fft_padx<NS>(j0);
+    constexpr int B = NS == 1 ? 32 : (NS == 8 ? 64 : 1);   // offsets that pad to a constant
 #pragma unroll
     for (int q = 0; q < 8 / R2; ++q)
 #pragma unroll
       for (int r = 0; r < R2; ++r) {
-        const int c = q * (N / 8) + r * (N / R2);   // a multiple of 8
-        a[q * R2 + r] = y[c + c / 8];
+        const int c = q * (N / 8) + r * (N / R2);
+        a[q * R2 + r] = c % B == 0 ? y[fft_padx<NS>(c)] : X[fft_padx<NS>(j0 + c)];
       }
     fft_from<N, NS * R, OFF + FftTw<N, NS>::here, NT>(a, j0, w, Y, X);
   }
