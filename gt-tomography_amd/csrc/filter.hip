@@ -582,7 +582,7 @@ __device__ __forceinline__ void dft4(cf &a0, cf &a1, cf &a2, cf &a3) {
   a3 = t1 - t3;
 }
 // y_s = sum_r a_r W8^(r s), W8 = exp(-i pi / 4): even and odd halves by dft4
-__device__ __forceinline__ void dft8(cf (&a)[8]) {
+__device__ __forceinline__ void dft8(cf *a) {
   cf e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6], o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
   dft4(e0, e1, e2, e3);
   dft4(o0, o1, o2, o3);
@@ -600,184 +600,242 @@ __device__ __forceinline__ void dft8(cf (&a)[8]) {
   a[7] = e3 - o3;
 }
 
+// y_s = sum_r a_r W16^(r s) as 4 x 4: t[n2][k1] = DFT4 over n1 of a[4 n1 + n2],
+// times W16^(n2 k1), then y[k1 + 4 k2] = DFT4 over n2 of t[n2][k1]
+__device__ __forceinline__ void dft16(cf *a) {
+  constexpr float C1 = 0.92387953251128676f, S1 = 0.38268343236508977f, R2 = 0.70710678118654752f;
+  cf t[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    cf x0 = a[n2], x1 = a[4 + n2], x2 = a[8 + n2], x3 = a[12 + n2];
+    dft4(x0, x1, x2, x3);
+    t[n2][0] = x0;
+    t[n2][1] = x1;
+    t[n2][2] = x2;
+    t[n2][3] = x3;
+  }
+  // W16^j for j = n2 k1: 1, 2, 3 (n2 = 1), 2, 4, 6 (n2 = 2), 3, 6, 9 (n2 = 3)
+  t[1][1] = cmul(t[1][1], {C1, -S1});
+  t[1][2] = cmul(t[1][2], {R2, -R2});
+  t[1][3] = cmul(t[1][3], {S1, -C1});
+  t[2][1] = cmul(t[2][1], {R2, -R2});
+  t[2][2] = mul_mi(t[2][2]);
+  t[2][3] = cmul(t[2][3], {-R2, -R2});
+  t[3][1] = cmul(t[3][1], {S1, -C1});
+  t[3][2] = cmul(t[3][2], {-R2, -R2});
+  t[3][3] = cmul(t[3][3], {-C1, S1});
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cf y0 = t[0][k1], y1 = t[1][k1], y2 = t[2][k1], y3 = t[3][k1];
+    dft4(y0, y1, y2, y3);
+    a[k1] = y0;
+    a[k1 + 4] = y1;
+    a[k1 + 8] = y2;
+    a[k1 + 12] = y3;
+  }
+}
+
 // LDS barrier: waits for this wave's LDS accesses only, so global loads
 // issued earlier (the next pair's rows) stay in flight across it
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-template <int N, int NS>
+// Each thread holds P points (16 for N >= 256, else 8); passes of radix P,
+// then one smaller radix pass
+template <int N, int NS, int P>
 struct FftRadix {
-  static constexpr int R = N / NS >= 8 ? 8 : N / NS;
+  static constexpr int R = N / NS >= P ? P : N / NS;
 };
 
-// twiddles a butterfly keeps: W^1, W^2, W^4 for radix 8 (W^3, W^5, W^6, W^7
-// are products of those, at most two deep), W^1, W^2 for radix 4, W^1 for 2
+// twiddles a butterfly keeps: W^1, W^2, W^4, W^8 below its radix (the other
+// powers are products of those, at most three deep)
 template <int R>
 struct FftKeep {
-  static constexpr int value = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  static constexpr int value = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
 };
 
-// twiddles one transform keeps per thread: (8/R) FftKeep<R> per pass with NS > 1
-template <int N, int NS>
+// twiddles one transform keeps per thread: (P/R) FftKeep<R> per pass with NS > 1
+template <int N, int NS, int P>
 struct FftTw {
-  static constexpr int R = FftRadix<N, NS>::R;
-  static constexpr int here = NS > 1 ? (8 / R) * FftKeep<R>::value : 0;
-  static constexpr int value = here + (NS * R < N ? FftTw<N, NS * R>::value : 0);
+  static constexpr int R = FftRadix<N, NS, P>::R;
+  static constexpr int here = NS > 1 ? (P / R) * FftKeep<R>::value : 0;
+  static constexpr int value = here + (NS * R < N ? FftTw<N, NS * R, P>::value : 0);
 };
-template <int N>
-struct FftTw<N, N> {
+template <int N, int P>
+struct FftTw<N, N, P> {
   static constexpr int here = 0, value = 0;
 };
 
 // the thread's kept twiddles from the table, in the order the passes use
-// them: pass (R, NS), butterfly j = j0 + q N/8: W_N^(p (j mod NS) N/(NS R))
-// for p = 1, 2, 4 (those below R)
-template <int N, int NS, int OFF, int NT>
+// them: pass (R, NS), butterfly j = j0 + q N/P: W_N^(p (j mod NS) N/(NS R))
+// for p = 1, 2, 4, 8 (those below R)
+template <int N, int NS, int P, int OFF, int NT>
 __device__ __forceinline__ void fft_tw_load(cf (&w)[NT], int j0, const cf *__restrict__ tw) {
-  constexpr int R = FftRadix<N, NS>::R, K = FftKeep<R>::value;
+  constexpr int R = FftRadix<N, NS, P>::R, K = FftKeep<R>::value;
   if constexpr (NS > 1) {
 #pragma unroll
-    for (int q = 0; q < 8 / R; ++q) {
-      const int k = (j0 + q * (N / 8)) & (NS - 1);
+    for (int q = 0; q < P / R; ++q) {
+      const int k = (j0 + q * (N / P)) & (NS - 1);
 #pragma unroll
       for (int i = 0; i < K; ++i) w[OFF + q * K + i] = tw[k * (1 << i) * (N / (NS * R))];
     }
   }
-  if constexpr (NS * R < N) fft_tw_load<N, NS * R, OFF + FftTw<N, NS>::here, NT>(w, j0, tw);
+  if constexpr (NS * R < N) fft_tw_load<N, NS * R, P, OFF + FftTw<N, NS, P>::here, NT>(w, j0, tw);
 }
 
 // keeps the derived twiddles' products inside the loop that uses them (the
-// compiler would otherwise hoist them and hold all seven per butterfly)
+// compiler would otherwise hoist them and hold all of them)
 __device__ __forceinline__ void fft_pin(cf &w) { asm volatile("" : "+v"(w.x), "+v"(w.y)); }
 
-// pass (R, NS) on the thread's butterflies j = j0 + q N/8, q < 8/R, whose
+// pass (R, NS) on the thread's butterflies j = j0 + q N/P, q < P/R, whose
 // inputs are a[q R + r]; the outputs replace them as a[q R + s]
-template <int N, int NS, int OFF, int NT>
-__device__ __forceinline__ void fft_pass(cf (&a)[8], cf (&w)[NT]) {
-  constexpr int R = FftRadix<N, NS>::R, K = FftKeep<R>::value;
+template <int N, int NS, int P, int OFF, int NT>
+__device__ __forceinline__ void fft_pass(cf (&a)[P], cf (&w)[NT]) {
+  constexpr int R = FftRadix<N, NS, P>::R, K = FftKeep<R>::value;
 #pragma unroll
-  for (int q = 0; q < 8 / R; ++q) {
+  for (int q = 0; q < P / R; ++q) {
+    cf *v = &a[q * R];
     if constexpr (NS > 1) {
       cf *wq = &w[OFF + q * K];
 #pragma unroll
       for (int i = 0; i < K; ++i) fft_pin(wq[i]);
-      if constexpr (R == 8) {
-        const cf w1 = wq[0], w2 = wq[1], w4 = wq[2], w3 = cmul(w1, w2);
-        a[1] = cmul(a[1], w1);
-        a[2] = cmul(a[2], w2);
-        a[3] = cmul(a[3], w3);
-        a[4] = cmul(a[4], w4);
-        a[5] = cmul(a[5], cmul(w1, w4));
-        a[6] = cmul(a[6], cmul(w2, w4));
-        a[7] = cmul(a[7], cmul(w3, w4));
-      } else if constexpr (R == 4) {
-        const cf w1 = wq[0], w2 = wq[1];
-        a[q * 4 + 1] = cmul(a[q * 4 + 1], w1);
-        a[q * 4 + 2] = cmul(a[q * 4 + 2], w2);
-        a[q * 4 + 3] = cmul(a[q * 4 + 3], cmul(w1, w2));
+      const cf w1 = wq[0];
+      if constexpr (R == 2) {
+        v[1] = cmul(v[1], w1);
       } else {
-        a[q * 2 + 1] = cmul(a[q * 2 + 1], wq[0]);
+        const cf w2 = wq[1], w3 = cmul(w1, w2);
+        v[1] = cmul(v[1], w1);
+        v[2] = cmul(v[2], w2);
+        v[3] = cmul(v[3], w3);
+        if constexpr (R >= 8) {
+          const cf w4 = wq[2], w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+          v[4] = cmul(v[4], w4);
+          v[5] = cmul(v[5], w5);
+          v[6] = cmul(v[6], w6);
+          v[7] = cmul(v[7], w7);
+          if constexpr (R == 16) {
+            const cf w8 = wq[3];
+            v[8] = cmul(v[8], w8);
+            v[9] = cmul(v[9], cmul(w1, w8));
+            v[10] = cmul(v[10], cmul(w2, w8));
+            v[11] = cmul(v[11], cmul(w3, w8));
+            v[12] = cmul(v[12], cmul(w4, w8));
+            v[13] = cmul(v[13], cmul(w5, w8));
+            v[14] = cmul(v[14], cmul(w6, w8));
+            v[15] = cmul(v[15], cmul(w7, w8));
+          }
+        }
       }
     }
-    if constexpr (R == 8) dft8(a);
-    else if constexpr (R == 4) dft4(a[q * 4], a[q * 4 + 1], a[q * 4 + 2], a[q * 4 + 3]);
-    else dft2(a[q * 2], a[q * 2 + 1]);
+    if constexpr (R == 16) dft16(v);
+    else if constexpr (R == 8) dft8(v);
+    else if constexpr (R == 4) dft4(v[0], v[1], v[2], v[3]);
+    else dft2(v[0], v[1]);
   }
 }
 
 // LDS padding of the exchange after pass NS (positions p < N, padded below
 // N + N/8): the pass's writes and the next pass's reads both fall on
 // distinct banks for every 32 lanes (64 banks of 4 bytes, 8-byte points).
-// After the first pass (writes 8 apart): p + p/32; after the second (groups
-// of 8 consecutive, 64 apart): p + 8 (p / 64); later passes write
+// After the first pass (writes P apart): p + p/32; after the second (groups
+// of P consecutive, P^2 apart): p + P (p / P^2); later passes write
 // consecutive positions: none.
-template <int NS>
+template <int NS, int P>
 __device__ __forceinline__ int fft_padx(int p) {
   if constexpr (NS == 1) return p + (p >> 5);
-  else if constexpr (NS == 8) return p + ((p >> 6) << 3);
+  else if constexpr (NS == P) return p + (p / (P * P)) * P;
   else return p;
 }
+template <int NS, int P>
+struct FftPadBlock {   // offsets that are multiples of this pad to a constant
+  static constexpr int value = NS == 1 ? 32 : (NS == P ? P * P : 1);
+};
 
 // passes from sub-transform size NS on; a[] holds that pass's inputs, and on
 // return the last pass's outputs.  Writes alternate between X and Y.  The
 // padded positions are written as one base register plus constant offsets
 // wherever the offsets allow it.
-template <int N, int NS, int OFF, int NT>
-__device__ __forceinline__ void fft_from(cf (&a)[8], int j0, cf (&w)[NT], cf *X, cf *Y) {
-  constexpr int R = FftRadix<N, NS>::R;
-  fft_pass<N, NS, OFF, NT>(a, w);
+template <int N, int NS, int P, int OFF, int NT>
+__device__ __forceinline__ void fft_from(cf (&a)[P], int j0, cf (&w)[NT], cf *X, cf *Y) {
+  constexpr int R = FftRadix<N, NS, P>::R;
+  fft_pass<N, NS, P, OFF, NT>(a, w);
   if constexpr (NS * R < N) {
-    constexpr int R2 = FftRadix<N, NS * R>::R;
-    if constexpr (NS == 1) {   // R = 8, one butterfly j0: positions 8 j0 + s
-      cf *x = X + 8 * j0 + (j0 >> 2);
+    constexpr int R2 = FftRadix<N, NS * R, P>::R;
+    if constexpr (NS == 1) {   // R = P, one butterfly j0: positions P j0 + s
+      cf *x = X + fft_padx<1, P>(P * j0);
 #pragma unroll
-      for (int s = 0; s < 8; ++s) x[s] = a[s];
-    } else if constexpr (NS == 8) {   // R = 8: positions 64 (j0 / 8) + j0 % 8 + 8 s
-      cf *x = X + 72 * (j0 >> 3) + (j0 & 7);
+      for (int s = 0; s < P; ++s) x[s] = a[s];
+    } else if constexpr (NS == P) {   // R = P: positions P^2 (j0 / P) + j0 % P + P s
+      cf *x = X + (P * P + P) * (j0 / P) + (j0 % P);
 #pragma unroll
-      for (int s = 0; s < 8; ++s) x[8 * s] = a[s];
+      for (int s = 0; s < P; ++s) x[P * s] = a[s];
     } else {
 #pragma unroll
-      for (int q = 0; q < 8 / R; ++q) {
-        const int j = j0 + q * (N / 8);
+      for (int q = 0; q < P / R; ++q) {
+        const int j = j0 + q * (N / P);
         cf *x = X + (j & ~(NS - 1)) * R + (j & (NS - 1));
 #pragma unroll
         for (int s = 0; s < R; ++s) x[s * NS] = a[q * R + s];
       }
     }
     lds_barrier();
-    const cf *y = X + fft_padx<NS>(j0);
-    constexpr int B = NS == 1 ? 32 : (NS == 8 ? 64 : 1);   // offsets that pad to a constant
+    const cf *y = X + fft_padx<NS, P>(j0);
+    constexpr int B = FftPadBlock<NS, P>::value;
 #pragma unroll
-    for (int q = 0; q < 8 / R2; ++q)
+    for (int q = 0; q < P / R2; ++q)
 #pragma unroll
       for (int r = 0; r < R2; ++r) {
-        const int c = q * (N / 8) + r * (N / R2);
-        a[q * R2 + r] = c % B == 0 ? y[fft_padx<NS>(c)] : X[fft_padx<NS>(j0 + c)];
+        const int c = q * (N / P) + r * (N / R2);
+        a[q * R2 + r] = c % B == 0 ? y[fft_padx<NS, P>(c)] : X[fft_padx<NS, P>(j0 + c)];
       }
-    fft_from<N, NS * R, OFF + FftTw<N, NS>::here, NT>(a, j0, w, Y, X);
+    fft_from<N, NS * R, P, OFF + FftTw<N, NS, P>::here, NT>(a, j0, w, Y, X);
   }
 }
 
-template <int N, int NS>
+template <int N, int NS, int P>
 struct FftWrites {   // LDS writes (passes - 1) of one transform
-  static constexpr int R = FftRadix<N, NS>::R;
-  static constexpr int value = NS * R < N ? 1 + FftWrites<N, NS * R>::value : 0;
+  static constexpr int R = FftRadix<N, NS, P>::R;
+  static constexpr int value = NS * R < N ? 1 + FftWrites<N, NS * R, P>::value : 0;
 };
-template <int N>
-struct FftWrites<N, N> {
+template <int N, int P>
+struct FftWrites<N, N, P> {
   static constexpr int value = 0;
 };
 
 template <int LOG2N>
 struct FftShape {
   static constexpr int N = 1 << LOG2N;
+  static constexpr int LP = LOG2N >= 8 ? 4 : 3;          // log2 of the points per thread
+  static constexpr int P = 1 << LP;
   static constexpr int G = N >= 2048 ? 1 : 2048 / N;   // row pairs per workgroup
-  static constexpr int T = G * N / 8;                  // threads
-  static constexpr int RL = N >> (3 * ((LOG2N - 1) / 3));   // radix of the last pass
+  static constexpr int T = G * N / P;                  // threads
+  static constexpr int RL = N >> (LP * ((LOG2N - 1) / LP));   // radix of the last pass
+  static constexpr int PN = N + N / 8;                 // padded length of one pair's buffer
 };
 
 // Persistent: workgroup b takes pair groups b, b + gridDim.x, ...; the next
 // group's rows (and cosine weights) are loaded while this one transforms.
+// The short-scan instantiation's weights need more registers than four waves
+// per SIMD allow without spills.
 template <int LOG2N, bool PARKER>
 __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
     const float *in, float *out, size_t nrows, int nu, const cf *__restrict__ tw,
     const float *__restrict__ cs, const float *__restrict__ cosw, int nv,
     const float *__restrict__ angles, float delta, float tan_delta) {
   using S = FftShape<LOG2N>;
-  constexpr int N = S::N, N8 = N / 8, G = S::G, T = S::T, RL = S::RL;
-  constexpr int PN = N + N / 8;   // padded length of one pair's buffer
+  constexpr int N = S::N, P = S::P, NP = N / P, G = S::G, T = S::T, RL = S::RL, PN = S::PN;
+  constexpr int H = P / 2;   // points j0 + m N/P with m >= P/2 are >= N/2 >= nu: never data
   constexpr int NW = T / 64 > 0 ? T / 64 : 1;
-  constexpr int NT = FftTw<N, 1>::value;
+  constexpr int NT = FftTw<N, 1, P>::value > 0 ? FftTw<N, 1, P>::value : 1;
   __shared__ cf buf[2][G * PN];
   __shared__ double part[2][NW];
-  const int t = threadIdx.x, g = t / N8, j0 = t & (N8 - 1);
+  const int t = threadIdx.x, g = t / NP, j0 = t & (NP - 1);
   const size_t ngroups = (nrows + 2 * G - 1) / (2 * G);
 
   cf w[NT];
-  fft_tw_load<N, 1, 0, NT>(w, j0, tw);
-  float csr[8];
+  fft_tw_load<N, 1, P, 0, NT>(w, j0, tw);
+  float csr[P];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) csr[m] = cs[j0 + m * N8];
+  for (int m = 0; m < P; ++m) csr[m] = cs[j0 + m * NP];
 
   // Rows go through buffer resources over one group's 2G rows (base and
   // size uniform, 32-bit offsets): reads past the group's last row or past
@@ -794,9 +852,8 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
   };
   constexpr unsigned OOB = 0x80000000u;   // beyond any group's bytes
   const __amdgpu_buffer_rsrc_t rcos = rsrc(cosw, cosw ? (size_t)nv * nu * sizeof(float) : 0);
-  // raw rows and cosine weights of group gi; positions j0 + m N/8 for m >= 4
-  // are >= N/2 >= nu
-  float x[2][4], cwv[2][4];
+  // raw rows and cosine weights of group gi
+  float x[2][H], cwv[2][H];
   auto fetch = [&](size_t gi) {
     const __amdgpu_buffer_rsrc_t rin =
         rsrc(in + gi * 2 * G * (size_t)nu, group_rows(gi) * nu * sizeof(float));
@@ -806,8 +863,8 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
       const int rl = 2 * g + h;   // row within the group
       const unsigned v = (v0 + (unsigned)rl) % (unsigned)nv;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int e = j0 + m * N8;
+      for (int m = 0; m < H; ++m) {
+        const int e = j0 + m * NP;
         const unsigned off = e < nu ? (unsigned)(rl * nu + e) * 4u : OOB;
         const unsigned coff = e < nu ? (v * (unsigned)nu + (unsigned)e) * 4u : OOB;
         x[h][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, off, 0, 0));
@@ -826,7 +883,7 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
     const __amdgpu_buffer_rsrc_t rout =
         rsrc(out + gi * 2 * G * (size_t)nu, group_rows(gi) * nu * sizeof(float));
     // the pair's weighted rows and their sums of squares
-    float xv[2][4];
+    float xv[2][H];
     double qs[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -835,23 +892,23 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
       if (PARKER && r < nrows) beta = angles[r / (size_t)nv] * (3.14159265358979f / 180.0f);
       double q = 0.0;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
+      for (int m = 0; m < H; ++m) {
         float v = x[h][m];
         if (cosw) v *= cwv[h][m];
-        if (PARKER) v *= parker_weight(j0 + m * N8, nu, beta, delta, tan_delta);
+        if (PARKER) v *= parker_weight(j0 + m * NP, nu, beta, delta, tan_delta);
         xv[h][m] = v;
         q += (double)v * (double)v;
       }
       qs[h] = q;
     }
     fetch(gi + gridDim.x);
-    // sums of squares over the pair's N/8 threads (within one wave up to 64)
+    // sums of squares over the pair's N/P threads (within one wave up to 64)
 #pragma unroll
-    for (int o = (N8 < 64 ? N8 : 64) / 2; o > 0; o >>= 1) {
+    for (int o = (NP < 64 ? NP : 64) / 2; o > 0; o >>= 1) {
       qs[0] += __shfl_xor(qs[0], o);
       qs[1] += __shfl_xor(qs[1], o);
     }
-    if constexpr (N8 > 64) {
+    if constexpr (NP > 64) {
       if ((t & 63) == 0) {
         part[0][t >> 6] = qs[0];
         part[1][t >> 6] = qs[1];
@@ -859,9 +916,9 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
       lds_barrier();
       qs[0] = qs[1] = 0.0;
 #pragma unroll
-      for (int wv = 0; wv < N8 / 64; ++wv) {
-        qs[0] += part[0][g * (N8 / 64) + wv];
-        qs[1] += part[1][g * (N8 / 64) + wv];
+      for (int wv = 0; wv < NP / 64; ++wv) {
+        qs[0] += part[0][g * (NP / 64) + wv];
+        qs[1] += part[1][g * (NP / 64) + wv];
       }
     }
     // power-of-two scales: |row / s| in [1, 2); zero rows give zero
@@ -878,34 +935,37 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
         sc[h] = inv[h] = 0.0f;
       }
     }
-    cf a[8];
+    cf a[P];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) a[m] = {xv[0][m] * inv[0], xv[1][m] * inv[1]};
+    for (int m = 0; m < H; ++m) a[m] = {xv[0][m] * inv[0], xv[1][m] * inv[1]};
 #pragma unroll
-    for (int m = 4; m < 8; ++m) a[m] = {0.0f, 0.0f};
+    for (int m = H; m < P; ++m) a[m] = {0.0f, 0.0f};
     cf *X = &buf[0][g * PN], *Y = &buf[1][g * PN];
-    fft_from<N, 1, 0, NT>(a, j0, w, X, Y);
-    // a[q RL + s] = Z[j0 + (q + s 8/RL) N/8]: the spectral product, conjugated,
-    // into the first pass's order b[m] = V[j0 + m N/8]
-    cf b[8];
+    fft_from<N, 1, P, 0, NT>(a, j0, w, X, Y);
+    // a[q RL + s] = Z[j0 + (q + s P/RL) N/P]: the spectral product, conjugated,
+    // into the first pass's order b[m] = V[j0 + m N/P]
+    cf b[P];
 #pragma unroll
-    for (int q = 0; q < 8 / RL; ++q)
+    for (int q = 0; q < P / RL; ++q)
 #pragma unroll
       for (int s = 0; s < RL; ++s) {
-        const int m = q + s * (8 / RL);
+        const int m = q + s * (P / RL);
         const cf z = a[q * RL + s];
         b[m] = {z.x * csr[m], -z.y * csr[m]};
       }
     // the next write goes to the buffer written last but one (a slower wave
     // may still read the last one); each transform writes FftWrites times,
     // so every group starts on X again
-    constexpr bool odd = FftWrites<N, 1>::value & 1;
-    fft_from<N, 1, 0, NT>(b, j0, w, odd ? Y : X, odd ? X : Y);
+    constexpr bool odd = FftWrites<N, 1, P>::value & 1;
+    fft_from<N, 1, P, 0, NT>(b, j0, w, odd ? Y : X, odd ? X : Y);
+    // outputs at j0 + m N/P; only m < P/2 can be below nu
 #pragma unroll
-    for (int q = 0; q < 8 / RL; ++q)
+    for (int q = 0; q < P / RL; ++q)
 #pragma unroll
       for (int s = 0; s < RL; ++s) {
-        const int e = j0 + (q + s * (8 / RL)) * N8;
+        const int m = q + s * (P / RL);
+        if (m >= H) continue;
+        const int e = j0 + m * NP;
         const cf v = b[q * RL + s];
         const unsigned oa = e < nu ? (unsigned)(2 * g * nu + e) * 4u : OOB;
         const unsigned ob = e < nu ? (unsigned)((2 * g + 1) * nu + e) * 4u : OOB;
