@@ -549,15 +549,16 @@ int run_toeplitz_mfma(cbct_ctx ctx, int nu, int nv, size_t nrows, const float *i
 // IFFT(Z C) = conj(FFT(conj(Z) C / N)), both transforms are forward ones:
 // v = FFT(conj(FFT(z)) C / N), y_a = s_a Re v, y_b = -s_b Im v.
 //
-// Each transform is a Stockham FFT: radix-8 passes, then one radix-2 or -4
-// pass; pass (R, NS) takes butterfly j's inputs X[j + r N/R], multiplies them
-// by W_N^(r (j mod NS) N / (NS R)), and writes its R outputs to
-// Y[(j / NS) NS R + j mod NS + s NS].  N/8 threads per row pair, 8 points per
-// thread; the passes exchange points through LDS (two buffers, one float2 of
-// padding after every 8).  A thread's 8 points sit at j0 + m N/8 both as the
-// first pass's inputs and as the last pass's outputs, so the loads, the
-// spectral product between the transforms and the stores stay in registers
-// and are coalesced.  One workgroup holds max(1, 2048 / N) pairs.
+// Each transform is a Stockham FFT: passes of radix P (the points a thread
+// holds: 16 for N >= 256, else 8), then one smaller radix pass; pass (R, NS)
+// takes butterfly j's inputs X[j + r N/R], multiplies them by
+// W_N^(r (j mod NS) N / (NS R)), and writes its R outputs to
+// Y[(j / NS) NS R + j mod NS + s NS].  N/P threads per row pair; the passes
+// exchange points through LDS (two buffers, padded per exchange).  A
+// thread's P points sit at j0 + m N/P both as the first pass's inputs and as
+// the last pass's outputs, so the loads, the spectral product between the
+// transforms and the stores stay in registers and are coalesced.  One
+// workgroup holds max(1, 2048 / N) pairs.
 struct cf {
   float x, y;
 };
