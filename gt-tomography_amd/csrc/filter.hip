@@ -647,14 +647,15 @@ struct FftRadix {
   static constexpr int R = N / NS >= P ? P : N / NS;
 };
 
-// twiddles a butterfly keeps: W^1, W^2, W^4, W^8 below its radix (the other
-// powers are products of those, at most three deep)
+// twiddles a butterfly keeps: all R - 1 (the kernel is bound by vector
+// instruction issue and holds two waves per SIMD, so registers are cheaper
+// than forming twiddles as products in every pass)
 template <int R>
 struct FftKeep {
-  static constexpr int value = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
+  static constexpr int value = R - 1;
 };
 
-// twiddles one transform keeps per thread: (P/R) FftKeep<R> per pass with NS > 1
+// twiddles one transform keeps per thread: (P/R)(R - 1) per pass with NS > 1
 template <int N, int NS, int P>
 struct FftTw {
   static constexpr int R = FftRadix<N, NS, P>::R;
@@ -666,9 +667,8 @@ struct FftTw<N, N, P> {
   static constexpr int here = 0, value = 0;
 };
 
-// the thread's kept twiddles from the table, in the order the passes use
-// them: pass (R, NS), butterfly j = j0 + q N/P: W_N^(p (j mod NS) N/(NS R))
-// for p = 1, 2, 4, 8 (those below R)
+// the thread's twiddles from the table, in the order the passes use them:
+// pass (R, NS), butterfly j = j0 + q N/P, input r: W_N^(r (j mod NS) N/(NS R))
 template <int N, int NS, int P, int OFF, int NT>
 __device__ __forceinline__ void fft_tw_load(cf (&w)[NT], int j0, const cf *__restrict__ tw) {
   constexpr int R = FftRadix<N, NS, P>::R, K = FftKeep<R>::value;
@@ -677,55 +677,23 @@ __device__ __forceinline__ void fft_tw_load(cf (&w)[NT], int j0, const cf *__res
     for (int q = 0; q < P / R; ++q) {
       const int k = (j0 + q * (N / P)) & (NS - 1);
 #pragma unroll
-      for (int i = 0; i < K; ++i) w[OFF + q * K + i] = tw[k * (1 << i) * (N / (NS * R))];
+      for (int r = 1; r < R; ++r) w[OFF + q * K + r - 1] = tw[k * r * (N / (NS * R))];
     }
   }
   if constexpr (NS * R < N) fft_tw_load<N, NS * R, P, OFF + FftTw<N, NS, P>::here, NT>(w, j0, tw);
 }
 
-// keeps the derived twiddles' products inside the loop that uses them (the
-// compiler would otherwise hoist them and hold all of them)
-__device__ __forceinline__ void fft_pin(cf &w) { asm volatile("" : "+v"(w.x), "+v"(w.y)); }
-
 // pass (R, NS) on the thread's butterflies j = j0 + q N/P, q < P/R, whose
 // inputs are a[q R + r]; the outputs replace them as a[q R + s]
 template <int N, int NS, int P, int OFF, int NT>
-__device__ __forceinline__ void fft_pass(cf (&a)[P], cf (&w)[NT]) {
+__device__ __forceinline__ void fft_pass(cf (&a)[P], const cf (&w)[NT]) {
   constexpr int R = FftRadix<N, NS, P>::R, K = FftKeep<R>::value;
 #pragma unroll
   for (int q = 0; q < P / R; ++q) {
     cf *v = &a[q * R];
     if constexpr (NS > 1) {
-      cf *wq = &w[OFF + q * K];
 #pragma unroll
-      for (int i = 0; i < K; ++i) fft_pin(wq[i]);
-      const cf w1 = wq[0];
-      if constexpr (R == 2) {
-        v[1] = cmul(v[1], w1);
-      } else {
-        const cf w2 = wq[1], w3 = cmul(w1, w2);
-        v[1] = cmul(v[1], w1);
-        v[2] = cmul(v[2], w2);
-        v[3] = cmul(v[3], w3);
-        if constexpr (R >= 8) {
-          const cf w4 = wq[2], w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
-          v[4] = cmul(v[4], w4);
-          v[5] = cmul(v[5], w5);
-          v[6] = cmul(v[6], w6);
-          v[7] = cmul(v[7], w7);
-          if constexpr (R == 16) {
-            const cf w8 = wq[3];
-            v[8] = cmul(v[8], w8);
-            v[9] = cmul(v[9], cmul(w1, w8));
-            v[10] = cmul(v[10], cmul(w2, w8));
-            v[11] = cmul(v[11], cmul(w3, w8));
-            v[12] = cmul(v[12], cmul(w4, w8));
-            v[13] = cmul(v[13], cmul(w5, w8));
-            v[14] = cmul(v[14], cmul(w6, w8));
-            v[15] = cmul(v[15], cmul(w7, w8));
-          }
-        }
-      }
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[OFF + q * K + r - 1]);
     }
     if constexpr (R == 16) dft16(v);
     else if constexpr (R == 8) dft8(v);
@@ -756,7 +724,7 @@ struct FftPadBlock {   // offsets that are multiples of this pad to a constant
 // padded positions are written as one base register plus constant offsets
 // wherever the offsets allow it.
 template <int N, int NS, int P, int OFF, int NT>
-__device__ __forceinline__ void fft_from(cf (&a)[P], int j0, cf (&w)[NT], cf *X, cf *Y) {
+__device__ __forceinline__ void fft_from(cf (&a)[P], int j0, const cf (&w)[NT], cf *X, cf *Y) {
   constexpr int R = FftRadix<N, NS, P>::R;
   fft_pass<N, NS, P, OFF, NT>(a, w);
   if constexpr (NS * R < N) {
