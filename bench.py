@@ -192,18 +192,43 @@ def kernel_rooflines(fp_kernel, fp_ms, bp_kernel, bp_ms, workload, nvox, nviews,
         bp.update({"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"})
     out["bp"] = bp
     if filt:
-        # the default engine (FFT): bound by HBM on its algorithmic bytes (each
-        # row read and written once); the MFMA engine against the MFMA peak
+        # the default engine (FFT): with its counters, the measured HBM bytes
+        # or VALU issue, whichever runs closer to its peak (as for BP); without
+        # them, HBM on its algorithmic bytes (each row read and written once).
+        # The MFMA engine beside it, against the MFMA peak.
         e2, e1 = filt["engine2"], filt["engine1"]
-        out["filter"] = {"kernel": e2["kernel"], "kernel_ms": e2["kernel_ms"], "bound": "hbm",
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "achieved": e2["gbs_algorithmic"],
-                         "frac": round(e2["gbs_algorithmic"] / HBM_PEAK_GBS, 4),
-                         "traffic": e2["bytes_algorithmic"],
-                         "mfma_engine": {"kernel": e1["kernel"], "kernel_ms": e1["kernel_ms"],
-                                         "bound": "mfma", "peak": FP16_MFMA_PEAK_TFLOPS,
-                                         "unit": "TFLOP/s", "achieved": e1["tflops_algorithmic"],
-                                         "frac": round(e1["tflops_algorithmic"] / FP16_MFMA_PEAK_TFLOPS, 4),
-                                         "issued_frac": e1["mfma_frac_of_2500_tflops"]}}
+        f_ms = e2["kernel_ms"]
+        fl = {"kernel": e2["kernel"], "kernel_ms": f_ms, "bound": "hbm", "peak": HBM_PEAK_GBS,
+              "unit": "GB/s", "achieved": e2["gbs_algorithmic"],
+              "frac": round(e2["gbs_algorithmic"] / HBM_PEAK_GBS, 4),
+              "hbm_frac_algorithmic": round(e2["gbs_algorithmic"] / HBM_PEAK_GBS, 4),
+              "traffic": e2["bytes_algorithmic"]}
+        pmc = read_pmc(e2["kernel"], workload, FILTER_SOURCES)
+        if pmc:
+            c = pmc["counters_per_launch"]
+            hbm = valu = None
+            if "bytes_per_launch" in pmc:
+                gbs = pmc["bytes_per_launch"] / (f_ms / 1e3) / 1e9
+                hbm = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(gbs / HBM_PEAK_GBS, 4)}
+                fl["traffic"] = pmc["bytes_per_launch"]
+                fl["hbm_frac_measured"] = hbm["frac"]
+            if "SQ_INSTS_VALU" in c:
+                tops = c["SQ_INSTS_VALU"] * 64 / (f_ms / 1e3) / 1e12
+                valu = {"achieved": round(tops, 2), "peak": VALU_PEAK_TOPS,
+                        "unit": "T VALU lane-instructions/s", "frac": round(tops / VALU_PEAK_TOPS, 4)}
+                fl["valu_frac"] = valu["frac"]
+            best = max((r for r in (("hbm", hbm), ("valu", valu)) if r[1]), key=lambda r: r[1]["frac"],
+                       default=None)
+            if best:
+                fl["bound"] = best[0]
+                fl.update(best[1])
+        fl["mfma_engine"] = {"kernel": e1["kernel"], "kernel_ms": e1["kernel_ms"], "bound": "mfma",
+                             "peak": FP16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "achieved": e1["tflops_algorithmic"],
+                             "frac": round(e1["tflops_algorithmic"] / FP16_MFMA_PEAK_TFLOPS, 4),
+                             "issued_frac": e1["mfma_frac_of_2500_tflops"]}
+        out["filter"] = fl
     return out
 
 
