@@ -1,7 +1,7 @@
 """The HIP path at the full sizes of the five standard workloads.
 
 * 128^3 FDK workload (180 views, 256 x 192 detector, Ram-Lak): FP, BP, FBP,
-  the FDK pre-filter on both filter engines and FDK_reconstruct_3d, all
+  the FDK pre-filter on all three filter engines and FDK_reconstruct_3d, all
   against the FP64 oracle at full size (rel-L2 <= 1e-5).
 * 256^3 OS workload (360 views, 512 x 384, 10 subsets, Poisson noise at
   I0 = 1e5): per-subset FP and BP against the oracle on a fixed subsample
