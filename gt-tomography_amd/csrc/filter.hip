@@ -797,6 +797,7 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
   constexpr int NT = FftTw<N, 1, P>::value > 0 ? FftTw<N, 1, P>::value : 1;
   __shared__ cf buf[2][G * PN];
   __shared__ double part[2][NW];
+  __shared__ float partf[2][NW];
   const int t = threadIdx.x, g = t / NP, j0 = t & (NP - 1);
   const size_t ngroups = (nrows + 2 * G - 1) / (2 * G);
 
@@ -851,43 +852,86 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
     const size_t ra = (gi * G + g) * 2;
     const __amdgpu_buffer_rsrc_t rout =
         rsrc(out + gi * 2 * G * (size_t)nu, group_rows(gi) * nu * sizeof(float));
-    // the pair's weighted rows and their sums of squares
+    // the pair's weighted rows
     float xv[2][H];
-    double qs[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const size_t r = ra + h;
       float beta = 0.0f;
       if (PARKER && r < nrows) beta = angles[r / (size_t)nv] * (3.14159265358979f / 180.0f);
-      double q = 0.0;
 #pragma unroll
       for (int m = 0; m < H; ++m) {
         float v = x[h][m];
         if (cosw) v *= cwv[h][m];
         if (PARKER) v *= parker_weight(j0 + m * NP, nu, beta, delta, tan_delta);
         xv[h][m] = v;
-        q += (double)v * (double)v;
       }
-      qs[h] = q;
     }
     fetch(gi + gridDim.x);
-    // sums of squares over the pair's N/P threads (within one wave up to 64)
+    // their sums of squares over the pair's N/P threads (within one wave up
+    // to 64): in FP32, and again in FP64 for a pair whose FP32 sums leave
+    // [2^-100, 2^120] (tiny, huge, zero or not finite rows).  Only a power of
+    // two near the norm is needed, so FP32 rounding does not matter; a pair
+    // that spans several waves is alone in its workgroup, so the FP64 pass's
+    // barrier is taken by the whole workgroup or not at all.
+    double qs[2];
+    {
+      float qf[2];
 #pragma unroll
-    for (int o = (NP < 64 ? NP : 64) / 2; o > 0; o >>= 1) {
-      qs[0] += __shfl_xor(qs[0], o);
-      qs[1] += __shfl_xor(qs[1], o);
-    }
-    if constexpr (NP > 64) {
-      if ((t & 63) == 0) {
-        part[0][t >> 6] = qs[0];
-        part[1][t >> 6] = qs[1];
+      for (int h = 0; h < 2; ++h) {
+        float q = 0.0f;
+#pragma unroll
+        for (int m = 0; m < H; ++m) q = fmaf(xv[h][m], xv[h][m], q);
+        qf[h] = q;
       }
-      lds_barrier();
-      qs[0] = qs[1] = 0.0;
 #pragma unroll
-      for (int wv = 0; wv < NP / 64; ++wv) {
-        qs[0] += part[0][g * (NP / 64) + wv];
-        qs[1] += part[1][g * (NP / 64) + wv];
+      for (int o = (NP < 64 ? NP : 64) / 2; o > 0; o >>= 1) {
+        qf[0] += __shfl_xor(qf[0], o);
+        qf[1] += __shfl_xor(qf[1], o);
+      }
+      if constexpr (NP > 64) {
+        if ((t & 63) == 0) {
+          partf[0][t >> 6] = qf[0];
+          partf[1][t >> 6] = qf[1];
+        }
+        lds_barrier();
+        qf[0] = qf[1] = 0.0f;
+#pragma unroll
+        for (int wv = 0; wv < NP / 64; ++wv) {
+          qf[0] += partf[0][g * (NP / 64) + wv];
+          qf[1] += partf[1][g * (NP / 64) + wv];
+        }
+      }
+      qs[0] = qf[0];
+      qs[1] = qf[1];
+      const bool in_range = qf[0] >= 0x1p-100f && qf[0] <= 0x1p120f && qf[1] >= 0x1p-100f &&
+                            qf[1] <= 0x1p120f;
+      if (!in_range) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          double q = 0.0;
+#pragma unroll
+          for (int m = 0; m < H; ++m) q += (double)xv[h][m] * (double)xv[h][m];
+          qs[h] = q;
+        }
+#pragma unroll
+        for (int o = (NP < 64 ? NP : 64) / 2; o > 0; o >>= 1) {
+          qs[0] += __shfl_xor(qs[0], o);
+          qs[1] += __shfl_xor(qs[1], o);
+        }
+        if constexpr (NP > 64) {
+          if ((t & 63) == 0) {
+            part[0][t >> 6] = qs[0];
+            part[1][t >> 6] = qs[1];
+          }
+          lds_barrier();
+          qs[0] = qs[1] = 0.0;
+#pragma unroll
+          for (int wv = 0; wv < NP / 64; ++wv) {
+            qs[0] += part[0][g * (NP / 64) + wv];
+            qs[1] += part[1][g * (NP / 64) + wv];
+          }
+        }
       }
     }
     // power-of-two scales: |row / s| in [1, 2); zero rows give zero
