@@ -457,6 +457,29 @@ def test_fft_filter_edges(ctx, nu, nrows):
     assert err <= TOL and err_in_place <= TOL
 
 
+def test_fft_filter_extreme_rows(ctx):
+    """Rows whose sums of squares leave the FP32 range (1e20: squares
+    overflow; 1e-25: squares underflow) take the FFT engine's FP64 norm
+    path, here at nu = 1024 where a row pair spans two waves; each row must
+    still meet the tolerance and a zero row stay zero."""
+    nu = 1024
+    print_case("fft filter extreme rows", nu)
+    rng = np.random.default_rng(5)
+    mags = [1e20, 1e-25, 1.0, 0.0, 3e-25, 1e20, 1e-3]
+    rows = rng.standard_normal((len(mags), nu)) * np.array(mags)[:, None]
+    F = O.guo_filter(nu)
+    ref = O.filter_rows(rows, F.astype(np.float32))
+    ctx.set_filter(F)
+    ctx.filter_engine(2)
+    out = torch.empty((len(mags), nu), device="cuda")
+    ctx.ramp_filter(nu, dev(rows), out)
+    got = host(out)
+    assert np.all(got[3] == 0.0)
+    errs = [rel_l2(got[r], ref[r]) for r in range(len(mags)) if mags[r] != 0.0]
+    print("fft filter extreme rows rel-L2", errs)
+    assert max(errs) <= TOL
+
+
 @pytest.mark.parametrize("engine", FILTER_ENGINES)
 @pytest.mark.parametrize("short_scan,offset", [(False, False), (True, False), (False, True)])
 def test_fdk_prefilter_parity(ctx, short_scan, offset, engine):
