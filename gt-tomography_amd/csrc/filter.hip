@@ -783,8 +783,8 @@ struct FftShape {
 
 // Persistent: workgroup b takes pair groups b, b + gridDim.x, ...; the next
 // group's rows (and cosine weights) are loaded while this one transforms.
-// The short-scan instantiation's weights need more registers than four waves
-// per SIMD allow without spills.
+// PARKER: the short-scan weights, in an instantiation of their own (their
+// range reductions would otherwise set the common kernel's register count).
 template <int LOG2N, bool PARKER>
 __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
     const float *in, float *out, size_t nrows, int nu, const cf *__restrict__ tw,
