@@ -831,7 +831,13 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int rl = 2 * g + h;   // row within the group
-      const unsigned v = (v0 + (unsigned)rl) % (unsigned)nv;
+      // v0 < nv and rl < 2G: one subtraction unless a view has fewer rows
+      unsigned v = v0 + (unsigned)rl;
+      if (2 * G <= nv) {
+        if (v >= (unsigned)nv) v -= (unsigned)nv;
+      } else {
+        v %= (unsigned)nv;
+      }
 #pragma unroll
       for (int m = 0; m < H; ++m) {
         const int e = j0 + m * NP;
@@ -934,14 +940,16 @@ __global__ void __launch_bounds__(FftShape<LOG2N>::T) k_ramp_fft(
         }
       }
     }
-    // power-of-two scales: |row / s| in [1, 2); zero rows give zero
+    // power-of-two scales: |row / s| in [1, 2); zero rows give zero.  With
+    // q = f 2^E (f in [1/2, 1)), s = 2^((E - 1) >> 1) puts q / s^2 in [1, 4)
     float sc[2], inv[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       if (qs[h] > 0.0) {
         int e = 0;
-        (void)frexp(sqrt(qs[h]), &e);
-        e = e - 1 < -120 ? -120 : (e - 1 > 120 ? 120 : e - 1);
+        (void)frexp(qs[h], &e);
+        e = (e - 1) >> 1;
+        e = e < -120 ? -120 : (e > 120 ? 120 : e);
         sc[h] = ldexpf(1.0f, e);
         inv[h] = ldexpf(1.0f, -e);
       } else {
