@@ -245,7 +245,9 @@ int cbct_default_filter(int nu, float *filter_host);
 /* Install the one-sided frequency response F[0..nu] (nu+1 floats, host).  The
  * context converts it to the equivalent spatial Toeplitz kernel of the
  * pad(2nu) -> R2C -> *F -> C2R -> /2nu -> crop chain, in fp64, and keeps the
- * fp32 kernel and its fp16 hi/lo split on the device.
+ * fp32 kernel on the device; the engines' own tables (the fp16 hi/lo split
+ * for the MFMA engine, twiddles and the kernel's spectrum for the FFT
+ * engine) are built from the fp64 kernel on first use.
  * Replaces set_frequency_filter (xray/hoCuConebeamProjectionOperator.h:117-119). */
 int cbct_filter_set(cbct_ctx ctx, int nu, const float *filter_host);
 
