@@ -480,6 +480,36 @@ def test_fft_filter_extreme_rows(ctx):
     assert max(errs) <= TOL
 
 
+@pytest.mark.parametrize("nu", [300, 1024, 2052])
+def test_fft_prefilter_short_scan_wide(ctx, nu):
+    """The FFT engine's short-scan (Parker) instantiation on the 16-point
+    transforms (N = 1024, 2048 and 8192: one, two and eight waves per row
+    pair), cosine and Parker weights fused into its loads, against the
+    oracle's pre-filter chain; in place as well.  At nu = 1024 all three
+    engines measure 3.6e-6: the FP32 Parker weights (shared by the engines)
+    against the oracle's, at the last view's steep edge; the same rows
+    filtered without them agree to 2e-7."""
+    g = geom(8, 8, 8, nu, 6, fov=(0.4 * nu, 2.4))
+    n = 9
+    angles = angles_full(n, 220.0)
+    print_case("fft prefilter short scan", nu)
+    proj = np.random.default_rng(nu).random(g.proj_shape(n))
+    F = O.guo_filter(nu)
+    ref = O.fdk_prefilter(to_oracle(g), angles, np.zeros((n, 2), np.float32), proj,
+                          F.astype(np.float32), short_scan=True, offset_correct=False)
+    ctx.set_views(angles)
+    ctx.set_filter(F)
+    ctx.filter_engine(2)
+    pin = dev(proj)
+    out = torch.empty_like(pin)
+    ctx.fdk_prefilter(g, pin, out, short_scan=True)
+    err = rel_l2(host(out), ref)
+    ctx.fdk_prefilter(g, pin, pin, short_scan=True)
+    err_in_place = rel_l2(host(pin), ref)
+    print("fft prefilter short scan rel-L2", nu, err, err_in_place)
+    assert err <= TOL and err_in_place <= TOL
+
+
 @pytest.mark.parametrize("engine", FILTER_ENGINES)
 @pytest.mark.parametrize("short_scan,offset", [(False, False), (True, False), (False, True)])
 def test_fdk_prefilter_parity(ctx, short_scan, offset, engine):
